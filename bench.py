@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: the 64-shot FP32 FWI gradient on one MI355X.
+
+Workload (one "step" = one full gradient): nx=2000, nz=400, h=25 m, dt=2 ms,
+nt=5000, 64 airgun shots with 600-1600-cell aperture windows, 20-cell CPML
+on all sides (free surface off), FP32 snapshots every 10 steps.  Each step
+runs forward modelling with snapshot storage for all shots, the L2 misfit and
+adjoint sources, the adjoint sweep with the imaging condition, and the FP64
+shot sum with the water mask -- all on the GPU, inputs resident in HBM.
+
+Prints one JSON line.  `value` is credited cell-updates per second
+(padded window cells x (nt-1) steps x 2 sweeps, summed over shots).
+`roofline` is the fused step kernel's algorithmic HBM bytes per launch
+divided by its average launch duration (HIP events on the context's stream).
+`cpu_baseline` times the reference's own Fortran solver (compiled from its
+sources into oracle/_ref) on a bounded sample of the same workload, one
+process per core as `mpirun -np C` over shots would run it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s fwd+adj (% HBM roofline); seconds per 64-shot FWI gradient"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nshots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=5000)
+    ap.add_argument("--vx", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--snap", choices=["f32", "bf16", "f16"], default="f32")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
+    ap.add_argument("--cpu-nt", type=int, default=200, help="time samples per CPU-baseline sweep")
+    return ap.parse_args()
+
+
+def frame_cells(acq, s):
+    """(padded cells, single-direction CPML cells, corner cells) of one shot."""
+    dn = acq.dpml + 6
+    ny, w = acq.ny, s["w"]
+    npad = (ny + 2 * dn) * (w + 2 * dn)
+    if acq.free_surface:
+        npad -= dn * (w + 2 * dn)          # the rows above the model are never updated
+        n1 = 2 * dn * ny + dn * w
+        n2 = 2 * dn * dn
+    else:
+        n1 = 2 * dn * ny + 2 * dn * w
+        n2 = 4 * dn * dn
+    return npad, n1, n2
+
+
+def algorithmic_bytes(acq, snap_bytes):
+    """Algorithmic HBM bytes of all step launches of one gradient (forward + adjoint):
+    16 B per updated cell (read p, p_prev, c2; write p_next), 16 B per active CPML
+    direction (read+write phi and chi), the snapshot write (forward) or read plus
+    FP32 imaging-kernel read-modify-write (adjoint) every store_snap steps."""
+    steps = acq.nt - 1
+    k = acq.store_snap
+    fwd = adj = 0.0
+    for s in acq.shots:
+        npad, n1, n2 = frame_cells(acq, s)
+        nm = acq.ny * s["w"]
+        base = 16.0 * npad + 16.0 * n1 + 32.0 * n2
+        fwd += steps * (base + snap_bytes * nm / k)
+        adj += steps * (base + (snap_bytes + 8.0) * nm / k)
+    return fwd + adj
+
+
+def _cpu_worker(args):
+    """One process: the reference solver, forward with snapshots then adjoint, on one window."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as rb
+    kind, vp, shot, wav, d, dt, nt = args
+    ss = 10
+    nts = nt // ss
+    t0 = time.perf_counter()
+    if kind == "reference":
+        R = rb.ref_solver("fs0_r4", 1, shot["src_x"], shot["src_y"], shot["rec_x"], shot["rec_y"], wav, None, d, dt,
+                          vp, ss, nts, nt)
+        adj = -R.rec[::-1].copy()
+        rb.ref_solver("fs0_r4", 2, shot["src_x"], shot["src_y"], shot["rec_x"], shot["rec_y"], wav, adj, d, dt, vp, ss,
+                      nts, nt, fields=R.fields)
+    else:
+        O = rb.oracle_solver("f32", 1, shot["src_x"], shot["src_y"], shot["rec_x"], shot["rec_y"], wav, None, d, dt,
+                             vp, ss, nts, nt, free_surface=0)
+        adj = -O.rec[::-1].copy()
+        rb.oracle_solver("f32", 2, shot["src_x"], shot["src_y"], shot["rec_x"], shot["rec_y"], wav, adj, d, dt, vp, ss,
+                         nts, nt, fields=O.fields, free_surface=0)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(acq, nproc, nt_sample):
+    """Reference solver on the host cores, one process per shot (as mpirun -np C over shots).
+    Runs before anything touches the GPU so the pool can fork."""
+    import multiprocessing as mp
+
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as rb
+    kind = "reference" if rb.ref_available("fs0_r4") else "port"
+    jobs, cells = [], 0
+    dn = acq.dpml + 6
+    for i in range(nproc):
+        s = acq.shots[i % len(acq.shots)]
+        vp = np.asfortranarray(acq.vp_init[:, s["x0"]:s["x0"] + s["w"]])
+        wav = np.asfortranarray(acq.wavelets[i % len(acq.wavelets)][:nt_sample])
+        jobs.append((kind, vp, s, wav, np.float32(acq.h), np.float32(acq.dt), nt_sample))
+        cells += (acq.ny + 2 * dn) * (s["w"] + 2 * dn) * (nt_sample - 1) * 2
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(nproc) as pool:
+        pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    rate = cells / wall / 1e9
+    full_cells = acq.cell_updates()
+    return {
+        "value": rate,
+        "unit": "Gcell-updates/s",
+        "cores": nproc,
+        "kind": kind,
+        "sample": (f"{nproc} config-2 shot windows (one per process), forward with snapshots + adjoint with "
+                   f"imaging, {nt_sample} time samples each; {cells:.3e} cell-updates in {wall:.1f} s"),
+        "seconds_per_gradient_est": full_cells / (rate * 1e9),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); the shot-parallel "
+              f"multi-GPU path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+    from fwi_amd import synth
+    acq = synth.config2(seed=0, nshots=a.nshots, nt=a.nt)
+    cpu = None
+    if not a.no_cpu:
+        nproc = a.cpu_procs or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(acq, nproc, a.cpu_nt)
+        print(f"[bench] cpu baseline: {cpu['value']:.4f} Gcell/s on {nproc} processes ({cpu['kind']})",
+              file=sys.stderr, flush=True)
+
+    import numpy as np
+
+    import fwi_amd
+    snap_kind = {"f32": fwi_amd.SNAP_F32, "bf16": fwi_amd.SNAP_BF16, "f16": fwi_amd.SNAP_F16}[a.snap]
+    snap_bytes = 4 if a.snap == "f32" else 2
+    ctx = fwi_amd.Context(acq.ny, acq.nx, np.float32(acq.h), np.float32(acq.dt), acq.nt, acq.store_snap,
+                          dpml=acq.dpml, free_surface=acq.free_surface, snap_kind=snap_kind, vx=a.vx,
+                          rows_per_task=a.rows)
+    t_setup = time.perf_counter()
+    ctx.set_shots(acq.shots)
+    ctx.set_sources(acq.wavelets)
+    ctx.set_model(acq.vp_true)
+    ctx.forward(store_snapshots=False)           # observed data from the true model
+    ctx.observed_from_synthetic()
+    ctx.set_model(acq.vp_init)
+    ctx.sync()
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s, device bytes {ctx.device_bytes() / 2**30:.1f} GiB",
+          file=sys.stderr, flush=True)
+    scale = 1.0 / len(acq.shots)
+    for i in range(a.warmup):
+        ctx.gradient(scale, acq.water_rows)
+        ctx.sync()
+        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    ctx.timing(reset=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    misfit = None
+    for i in range(a.steps):
+        misfit = ctx.gradient(scale, acq.water_rows)
+        print(f"[bench] step {i + 1}/{a.steps} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    step_ms, nlaunch = ctx.timing()
+    grad = ctx.get_gradient()
+    assert np.isfinite(grad).all() and np.abs(grad).max() > 0, "gradient is not finite / is zero"
+
+    cells = acq.cell_updates() * a.steps
+    gcell = cells / wall / 1e9
+    bytes_per_grad = algorithmic_bytes(acq, snap_bytes)
+    launches_per_grad = 2 * (acq.nt - 1)
+    avg_launch_s = step_ms / 1e3 / max(nlaunch, 1)
+    bytes_per_launch = bytes_per_grad / launches_per_grad
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": gcell,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{len(acq.shots)}-shot FWI gradient (forward+snapshots, L2 misfit, adjoint+imaging, FP64 "
+                         f"shot sum): nx=2000 nz=400 h=25m dt=2ms nt={acq.nt}, windows 600-1600 cells, CPML 20 all "
+                         f"sides, {a.snap} snapshots every {acq.store_snap} steps, seed 0"),
+            "nshots": len(acq.shots),
+            "nt": acq.nt,
+            "snapshots": a.snap,
+            "vx": a.vx or 4,
+            "rows_per_task": a.rows or 64,
+        },
+        "seconds_per_gradient": wall / a.steps,
+        "misfit": misfit,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "step_kernel (fused stencil + CPML + injection + receivers + snapshot/imaging)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": avg_launch_s * 1e6,
+            "launches": nlaunch,
+            "frac_of_wall_in_step_kernel": (step_ms / 1e3) / wall,
+        },
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (gcell / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

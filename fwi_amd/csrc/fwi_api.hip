@@ -1,0 +1,800 @@
+// Host side of libfwi_hip.so: batched shot context, drop-in solver symbols.
+//
+// A context (fwi_ctx) holds every shot of a batch resident on one GPU: its
+// padded c2, two wavefield time levels, CPML memory variables, snapshots,
+// per-shot imaging kernel and traces.  One launch of the fused step kernel
+// advances all shots by one time step.  The drop-in symbols
+// (solver_acoustic_ss_, solver_forward_simple_) run a one-shot context and
+// translate between the reference's host array layouts and the device
+// layouts.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fwi.h"
+#include "fwi_aux.h"
+#include "fwi_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& m) {
+    g_err = m;
+    return -1;
+}
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+#define CHK(x)                       \
+    do {                             \
+        int r_ = (x);                \
+        if (r_ != 0) return r_;      \
+    } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// CPML factors A(m), B(m) for the 1-based distance m into the frame
+// (solver_PML_coeffs.f90:16-34, evaluated in REAL(4) like the reference):
+// zero for m <= acc, the damped profile for acc < m <= dpml+acc, and the
+// last damped value repeated outward.  The corner matrices of the reference
+// are f(max(dx, dy)) of this table (solver_PML_coeffs.f90:36-70).
+void cpml_table(int dpml, float dt, float d, int dn, float* fA, float* fB) {
+    const int acc = fwi::ACC;
+    const float pi = 3.14159265f;
+    const float refl = 0.00001f;
+    const float w = float(dpml - 1) * d;
+    for (int m = 0; m <= dn; ++m) { fA[m] = 0.0f; fB[m] = 0.0f; }
+    for (int m = acc + 1; m <= dpml + acc; ++m) {
+        const float y = float(m - (acc + 1)) * d;
+        const float q = y / w;
+        const float alpha = (30.0f * pi) * (1.0f - q);
+        const float sigma = (-((6000.0f * std::log(refl)) / (2.0f * w))) * (q * q);
+        fA[m] = std::exp(-((sigma + alpha) * dt));
+        fB[m] = ((fA[m] - 1.0f) * sigma) / (sigma + alpha);
+    }
+    for (int m = dpml + acc + 1; m <= dn; ++m) { fA[m] = fA[dpml + acc]; fB[m] = fB[dpml + acc]; }
+}
+
+// One device allocation carved into aligned pieces.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t add(size_t bytes) {
+        const size_t off = size;
+        size = align_up(size + bytes, 256);
+        return off;
+    }
+    template <class T> T* at(size_t off) const { return reinterpret_cast<T*>(base + off); }
+    int alloc() {
+        if (size == 0) return 0;
+        hipError_t e = hipMalloc(&base, size);
+        if (e != hipSuccess) return fail("hipMalloc(" + std::to_string(size) + " bytes): " + hipGetErrorString(e));
+        return 0;
+    }
+    void release() {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        size = 0;
+    }
+};
+
+struct Shot {
+    int x0 = 0, W = 0;
+    std::vector<int> sx, sy, rx, ry;     // 1-based window coordinates
+    int NX = 0, P = 0, Ws = 0, ntile = 0;
+    std::vector<int> txs, txe;           // tile column ranges
+    // CSR tables (host), built per tiling
+    std::vector<int> iptr[2], rptr;
+    std::vector<int2> ient[2], rent;
+    // arena offsets
+    size_t o_p[2], o_phx[2], o_chx, o_phy[2], o_chy;                    // state arena
+    size_t o_c2, o_iptr[2], o_ient[2], o_rptr, o_rent;                  // static arena
+    size_t o_src, o_rec, o_obs, o_adj;                                  // trace arena
+    size_t o_snap, o_grad;                                              // snapshot arena / grad arena
+    size_t toff = 0;                                                    // receiver-trace element offset
+    size_t soff = 0;                                                    // source-trace element offset
+    int roff = 0;                                                       // first global receiver index
+};
+
+}  // namespace
+
+struct fwi_ctx {
+    fwi_config cfg{};
+    int dn = 0, NY = 0, nt_store = 0, vx = 4, rows = 64;
+    float fA[fwi::MAXDN + 1], fB[fwi::MAXDN + 1];
+    std::vector<Shot> shots;
+    int nrec_total = 0, nsrc_total = 0;
+    Arena state, stat, traces, snaps, grads, misc;
+    fwi::ShotDev* d_desc = nullptr;
+    fwi::Task* d_tasks = nullptr;
+    int ntasks = 0;
+    float* d_vp = nullptr;
+    fwi::C2Shot* d_c2shots = nullptr;
+    fwi::RecMap* d_recmap = nullptr;
+    fwi::ResShot* d_resshots = nullptr;
+    fwi::GradShot* d_gradshots = nullptr;
+    float* d_invs = nullptr;
+    double* d_partial = nullptr;
+    int nblk_misfit = 0;
+    double* d_shot_misfit = nullptr;
+    double* d_grad = nullptr;
+    int* d_water = nullptr;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;   // step-kernel spans not yet read
+    double step_ms = 0.0;
+    long long step_launches = 0;
+    bool have_shots = false, have_model = false, have_synth = false, have_snaps = false;
+
+    ~fwi_ctx() {
+        release();
+        for (auto& e : ev_pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void release() {
+        state.release(); stat.release(); traces.release(); snaps.release(); grads.release(); misc.release();
+        have_shots = have_model = have_synth = have_snaps = false;
+    }
+    size_t snap_elem() const { return cfg.snap_kind == fwi::SNAP_F32 ? 4 : 2; }
+};
+
+namespace {
+
+int validate_config(const fwi_config& c) {
+    if (c.ny < 1 || c.nx < 1) return fail("model dimensions must be positive");
+    if (c.nt < 2) return fail("nt must be at least 2");
+    if (c.store_snap < 1) return fail("store_snap must be positive");
+    if (c.dpml < 1 || c.dpml + 2 * fwi::ACC > fwi::MAXDN) return fail("dpml out of range");
+    if (!(c.d > 0.0f) || !(c.dt > 0.0f)) return fail("d and dt must be positive");
+    if (c.snap_kind != fwi::SNAP_F32 && c.snap_kind != fwi::SNAP_BF16 && c.snap_kind != fwi::SNAP_F16)
+        return fail("unknown snapshot kind");
+    if (c.vx != 0 && c.vx != 1 && c.vx != 2 && c.vx != 4) return fail("vx must be 1, 2 or 4");
+    return 0;
+}
+
+// Split the padded width into tiles of at most step_tile_cols(vx) columns.
+// Every tile starts at a multiple of 4 and at or before column NX-4, which
+// keeps the clamped stencil of the last columns inside the tile's LDS row.
+void make_tiles(Shot& s, int vx) {
+    const int tw = fwi::step_tile_cols(vx);
+    s.ntile = (s.NX + tw - 1) / tw;
+    s.txs.assign(s.ntile, 0);
+    s.txe.assign(s.ntile, 0);
+    for (int t = 0; t < s.ntile; ++t) s.txs[t] = t * tw;
+    if (s.ntile > 1 && s.txs[s.ntile - 1] > s.NX - 4) s.txs[s.ntile - 1] = (s.NX - 4) & ~3;
+    for (int t = 0; t < s.ntile; ++t) s.txe[t] = (t + 1 < s.ntile) ? s.txs[t + 1] : s.NX;
+}
+
+int tile_of(const Shot& s, int X) {
+    for (int t = 0; t < s.ntile; ++t)
+        if (X >= s.txs[t] && X < s.txe[t]) return t;
+    return -1;
+}
+
+// CSR over (row, tile) of cells with a trace index each.
+void build_csr(const Shot& s, int NY, const std::vector<std::pair<int, int>>& cells /* (Y*NX+X) */,
+               const std::vector<int>& trace, std::vector<int>& ptr, std::vector<int2>& ent) {
+    const int nc = NY * s.ntile;
+    ptr.assign(nc + 1, 0);
+    std::vector<int> cnt(nc, 0);
+    std::vector<int> key(cells.size());
+    for (size_t e = 0; e < cells.size(); ++e) {
+        const int Y = cells[e].first, X = cells[e].second;
+        key[e] = Y * s.ntile + tile_of(s, X);
+        cnt[key[e]]++;
+    }
+    for (int c = 0; c < nc; ++c) ptr[c + 1] = ptr[c] + cnt[c];
+    ent.assign(cells.size(), int2{0, 0});
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t e = 0; e < cells.size(); ++e) ent[fill[key[e]]++] = int2{cells[e].second, trace[e]};
+}
+
+int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
+    c->release();
+    const fwi_config& cf = c->cfg;
+    const int dn = c->dn, NY = c->NY, ny = cf.ny;
+    c->shots.assign(nshots, Shot());
+    c->nrec_total = c->nsrc_total = 0;
+    for (int i = 0; i < nshots; ++i) {
+        Shot& s = c->shots[i];
+        const fwi_shot& f = in[i];
+        if (f.w < 1 || f.x0 < 0 || f.x0 + f.w > cf.nx)
+            return fail("shot " + std::to_string(i) + ": window outside the model");
+        if (f.nsrc < 0 || f.nrec < 0) return fail("negative source/receiver count");
+        s.x0 = f.x0;
+        s.W = f.w;
+        s.sx.assign(f.src_x, f.src_x + f.nsrc);
+        s.sy.assign(f.src_y, f.src_y + f.nsrc);
+        s.rx.assign(f.rec_x, f.rec_x + f.nrec);
+        s.ry.assign(f.rec_y, f.rec_y + f.nrec);
+        s.NX = s.W + 2 * dn;
+        s.P = (int)align_up(s.NX, 64);
+        s.Ws = (int)align_up(s.W, 4);
+        make_tiles(s, c->vx);
+        auto in_grid = [&](int x, int y) { return x >= 1 - dn && x <= s.W + dn && y >= 1 - dn && y <= ny + dn; };
+        for (size_t j = 0; j < s.sx.size(); ++j)
+            if (!in_grid(s.sx[j], s.sy[j])) return fail("shot " + std::to_string(i) + ": source outside the padded grid");
+        for (size_t j = 0; j < s.rx.size(); ++j)
+            if (!in_grid(s.rx[j], s.ry[j])) return fail("shot " + std::to_string(i) + ": receiver outside the padded grid");
+        // injection tables: cells inside the model only (the reference writes
+        // its source array everywhere but only the interior update reads it);
+        // a cell listed twice keeps its last trace (solver_acoustic_SS.f90:320-329)
+        for (int dir = 0; dir < 2; ++dir) {
+            const std::vector<int>& X = dir == 0 ? s.sx : s.rx;
+            const std::vector<int>& Y = dir == 0 ? s.sy : s.ry;
+            std::map<std::pair<int, int>, int> last;
+            for (size_t j = 0; j < X.size(); ++j)
+                if (X[j] >= 1 && X[j] <= s.W && Y[j] >= 1 && Y[j] <= ny) last[{dn + Y[j] - 1, dn + X[j] - 1}] = (int)j;
+            std::vector<std::pair<int, int>> cells;
+            std::vector<int> tr;
+            for (auto& kv : last) { cells.push_back(kv.first); tr.push_back(kv.second); }
+            build_csr(s, NY, cells, tr, s.iptr[dir], s.ient[dir]);
+        }
+        {
+            std::vector<std::pair<int, int>> cells;
+            std::vector<int> tr;
+            for (size_t j = 0; j < s.rx.size(); ++j) { cells.push_back({dn + s.ry[j] - 1, dn + s.rx[j] - 1}); tr.push_back((int)j); }
+            build_csr(s, NY, cells, tr, s.rptr, s.rent);
+        }
+        s.roff = c->nrec_total;
+        s.toff = (size_t)c->nrec_total * cf.nt;
+        s.soff = (size_t)c->nsrc_total * cf.nt;
+        c->nrec_total += (int)s.rx.size();
+        c->nsrc_total += (int)s.sx.size();
+    }
+
+    // arena layout
+    for (auto& s : c->shots) {
+        const size_t field_bytes = (4 + (size_t)NY * s.P) * sizeof(float);
+        for (int b = 0; b < 2; ++b) s.o_p[b] = c->state.add(field_bytes);
+        const size_t xb = (size_t)2 * NY * dn * sizeof(float), yb = (size_t)2 * dn * s.P * sizeof(float);
+        for (int b = 0; b < 2; ++b) s.o_phx[b] = c->state.add(xb);
+        s.o_chx = c->state.add(xb);
+        for (int b = 0; b < 2; ++b) s.o_phy[b] = c->state.add(yb);
+        s.o_chy = c->state.add(yb);
+        s.o_c2 = c->stat.add(field_bytes);
+        for (int d = 0; d < 2; ++d) {
+            s.o_iptr[d] = c->stat.add(s.iptr[d].size() * sizeof(int));
+            s.o_ient[d] = c->stat.add(std::max<size_t>(1, s.ient[d].size()) * sizeof(int2));
+        }
+        s.o_rptr = c->stat.add(s.rptr.size() * sizeof(int));
+        s.o_rent = c->stat.add(std::max<size_t>(1, s.rent.size()) * sizeof(int2));
+        s.o_snap = c->snaps.add((size_t)(c->nt_store + 1) * ny * s.Ws * c->snap_elem());
+        s.o_grad = c->grads.add((size_t)ny * s.Ws * sizeof(float));
+    }
+    const size_t ntr_rec = std::max<size_t>(1, (size_t)c->nrec_total * cf.nt);
+    const size_t ntr_src = std::max<size_t>(1, (size_t)c->nsrc_total * cf.nt);
+    const size_t o_src = c->traces.add(ntr_src * sizeof(float));
+    const size_t o_rec = c->traces.add(ntr_rec * sizeof(float));
+    const size_t o_obs = c->traces.add(ntr_rec * sizeof(float));
+    const size_t o_adj = c->traces.add(ntr_rec * sizeof(float));
+    for (auto& s : c->shots) {
+        s.o_src = o_src + s.soff * sizeof(float);
+        s.o_rec = o_rec + s.toff * sizeof(float);
+        s.o_obs = o_obs + s.toff * sizeof(float);
+        s.o_adj = o_adj + s.toff * sizeof(float);
+    }
+
+    // tasks: per shot, row chunks of about `rows` rows, then tiles
+    const int y0 = cf.free_surface ? dn : 0;
+    std::vector<fwi::Task> tasks;
+    for (int si = 0; si < nshots; ++si) {
+        const Shot& s = c->shots[si];
+        const int nrows = NY - y0;
+        const int nch = std::max(1, (nrows + c->rows - 1) / c->rows);
+        for (int ch = 0; ch < nch; ++ch) {
+            const int ya = y0 + (int)((long long)nrows * ch / nch), yb = y0 + (int)((long long)nrows * (ch + 1) / nch);
+            for (int t = 0; t < s.ntile; ++t) tasks.push_back(fwi::Task{si, s.txs[t], s.txe[t], ya, yb, t, 0, 0});
+        }
+    }
+    c->ntasks = (int)tasks.size();
+
+    // misc arena: descriptors, tasks, misfit scratch, gradient
+    const int nblk = 64;
+    c->nblk_misfit = nblk;
+    const size_t o_desc = c->misc.add(nshots * sizeof(fwi::ShotDev));
+    const size_t o_tasks = c->misc.add(std::max<size_t>(1, tasks.size()) * sizeof(fwi::Task));
+    const size_t o_vp = c->misc.add((size_t)cf.ny * cf.nx * sizeof(float));
+    const size_t o_c2s = c->misc.add(nshots * sizeof(fwi::C2Shot));
+    const size_t o_rmap = c->misc.add(std::max(1, c->nrec_total) * sizeof(fwi::RecMap));
+    const size_t o_rs = c->misc.add(nshots * sizeof(fwi::ResShot));
+    const size_t o_gs = c->misc.add(nshots * sizeof(fwi::GradShot));
+    const size_t o_invs = c->misc.add(std::max(1, c->nrec_total) * 2 * sizeof(float));
+    const size_t o_part = c->misc.add((size_t)nshots * nblk * sizeof(double));
+    const size_t o_sm = c->misc.add(nshots * sizeof(double));
+    const size_t o_grad = c->misc.add((size_t)cf.ny * cf.nx * sizeof(double));
+    const size_t o_water = c->misc.add(cf.nx * sizeof(int));
+
+    CHK(c->state.alloc());
+    CHK(c->stat.alloc());
+    CHK(c->traces.alloc());
+    CHK(c->snaps.alloc());
+    CHK(c->grads.alloc());
+    CHK(c->misc.alloc());
+    HIPCHK(hipMemset(c->stat.base, 0, c->stat.size));
+
+    c->d_desc = c->misc.at<fwi::ShotDev>(o_desc);
+    c->d_tasks = c->misc.at<fwi::Task>(o_tasks);
+    c->d_vp = c->misc.at<float>(o_vp);
+    c->d_c2shots = c->misc.at<fwi::C2Shot>(o_c2s);
+    c->d_recmap = c->misc.at<fwi::RecMap>(o_rmap);
+    c->d_resshots = c->misc.at<fwi::ResShot>(o_rs);
+    c->d_gradshots = c->misc.at<fwi::GradShot>(o_gs);
+    c->d_invs = c->misc.at<float>(o_invs);
+    c->d_partial = c->misc.at<double>(o_part);
+    c->d_shot_misfit = c->misc.at<double>(o_sm);
+    c->d_grad = c->misc.at<double>(o_grad);
+    c->d_water = c->misc.at<int>(o_water);
+
+    // descriptors
+    std::vector<fwi::ShotDev> desc(nshots);
+    std::vector<fwi::C2Shot> c2s(nshots);
+    std::vector<fwi::RecMap> rmap(std::max(1, c->nrec_total));
+    std::vector<fwi::ResShot> rs(nshots);
+    std::vector<fwi::GradShot> gs(nshots);
+    for (int si = 0; si < nshots; ++si) {
+        Shot& s = c->shots[si];
+        fwi::ShotDev& d = desc[si];
+        std::memset(&d, 0, sizeof(d));
+        d.NX = s.NX; d.P = s.P; d.W = s.W; d.Ws = s.Ws; d.ntile = s.ntile;
+        d.ntr_src = (int)s.sx.size(); d.nrec = (int)s.rx.size();
+        for (int b = 0; b < 2; ++b) {
+            d.p[b] = c->state.at<float>(s.o_p[b]) + 4;
+            d.phx[b] = c->state.at<float>(s.o_phx[b]);
+            d.phy[b] = c->state.at<float>(s.o_phy[b]);
+        }
+        d.chx = c->state.at<float>(s.o_chx);
+        d.chy = c->state.at<float>(s.o_chy);
+        d.c2 = c->stat.at<float>(s.o_c2) + 4;
+        for (int k = 0; k < 2; ++k) {
+            d.inj_ptr[k] = c->stat.at<int>(s.o_iptr[k]);
+            d.inj_ent[k] = c->stat.at<int2>(s.o_ient[k]);
+        }
+        d.inj_tr[0] = c->traces.at<float>(s.o_src);
+        d.inj_tr[1] = c->traces.at<float>(s.o_adj);
+        d.ntr[0] = (int)s.sx.size();
+        d.ntr[1] = (int)s.rx.size();
+        d.rec_ptr = c->stat.at<int>(s.o_rptr);
+        d.rec_ent = c->stat.at<int2>(s.o_rent);
+        d.rec_tr = c->traces.at<float>(s.o_rec);
+        d.snap = c->snaps.at<void>(s.o_snap);
+        d.snap_scale = nullptr;
+        d.grad = c->grads.at<float>(s.o_grad);
+        d.snap_stride = (size_t)cf.ny * s.Ws;
+        c2s[si] = fwi::C2Shot{const_cast<float*>(d.c2), s.NX, s.P, s.W, s.x0};
+        for (int r = 0; r < (int)s.rx.size(); ++r) rmap[s.roff + r] = fwi::RecMap{s.toff, r, (int)s.rx.size()};
+        rs[si] = fwi::ResShot{s.toff, s.roff, (int)s.rx.size()};
+        gs[si] = fwi::GradShot{d.grad, s.x0, s.W, s.Ws, 0};
+        for (int k = 0; k < 2; ++k) {
+            HIPCHK(hipMemcpy(c->stat.base + s.o_iptr[k], s.iptr[k].data(), s.iptr[k].size() * sizeof(int), hipMemcpyHostToDevice));
+            if (!s.ient[k].empty())
+                HIPCHK(hipMemcpy(c->stat.base + s.o_ient[k], s.ient[k].data(), s.ient[k].size() * sizeof(int2), hipMemcpyHostToDevice));
+        }
+        HIPCHK(hipMemcpy(c->stat.base + s.o_rptr, s.rptr.data(), s.rptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!s.rent.empty())
+            HIPCHK(hipMemcpy(c->stat.base + s.o_rent, s.rent.data(), s.rent.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->d_desc, desc.data(), nshots * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
+    if (!tasks.empty()) HIPCHK(hipMemcpy(c->d_tasks, tasks.data(), tasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_c2shots, c2s.data(), nshots * sizeof(fwi::C2Shot), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_recmap, rmap.data(), rmap.size() * sizeof(fwi::RecMap), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_resshots, rs.data(), nshots * sizeof(fwi::ResShot), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_gradshots, gs.data(), nshots * sizeof(fwi::GradShot), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->traces.base, 0, c->traces.size));
+    HIPCHK(hipMemset(c->snaps.base, 0, c->snaps.size));
+    HIPCHK(hipMemset(c->grads.base, 0, c->grads.size));
+    c->have_shots = true;
+    return 0;
+}
+
+// Time loop: k = 2..nt as in solver_acoustic_SS.f90:313.
+// mode 0 forward (record receivers; store snapshots if store), 1 adjoint
+// (inject adjoint sources at receivers; imaging condition).
+int run(fwi_ctx* c, int mode, bool store) {
+    const fwi_config& cf = c->cfg;
+    HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+    fwi::StepParams prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.shots = c->d_desc;
+    prm.tasks = c->d_tasks;
+    prm.ntasks = c->ntasks;
+    prm.NY = c->NY;
+    prm.dn = c->dn;
+    prm.ny = cf.ny;
+    prm.mode = mode;
+    prm.record = mode == 0 ? 1 : 0;
+    prm.snap_kind = cf.snap_kind;
+    prm.dt = cf.dt;
+    prm.ssf = (float)cf.store_snap;
+    prm.free_surface = cf.free_surface;
+    std::memcpy(prm.fA, c->fA, sizeof(prm.fA));
+    std::memcpy(prm.fB, c->fB, sizeof(prm.fB));
+    if (mode == 0) {
+        const size_t ntr = (size_t)c->nrec_total * cf.nt * sizeof(float);
+        if (ntr) HIPCHK(hipMemsetAsync(c->traces.base + c->shots[0].o_rec, 0, ntr, c->stream));
+    }
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        HIPCHK(hipEventRecord(ev0, c->stream));
+    }
+    for (int k = 2; k <= cf.nt; ++k) {
+        prm.k = k;
+        prm.snap_op = 0;
+        prm.slot = 0;
+        if (k % cf.store_snap == 0) {
+            const int m = k / cf.store_snap;
+            if (mode == 0 && store) {
+                if (m > c->nt_store) return fail("snapshot slot overflow (nt_Store too small)");
+                prm.snap_op = 1;
+                prm.slot = m;
+            } else if (mode == 1) {
+                const int slot = c->nt_store - m;
+                if (slot < 0) return fail("snapshot slot underflow (nt_Store too small)");
+                prm.snap_op = 2;
+                prm.slot = slot;
+            }
+        }
+        HIPCHK(fwi::launch_step(c->vx, prm, c->stream));
+    }
+    if (c->timing) {
+        HIPCHK(hipEventRecord(ev1, c->stream));
+        c->ev_pending.push_back({ev0, ev1});
+        c->step_launches += cf.nt - 1;
+    }
+    if (mode == 0) {
+        // Rec(nt,:) = Rec(nt-1,:)  (solver_acoustic_SS.f90:588)
+        for (auto& s : c->shots) {
+            const size_t n = s.rx.size();
+            if (!n) continue;
+            float* base = c->traces.at<float>(s.o_rec);
+            HIPCHK(hipMemcpyAsync(base + (size_t)(cf.nt - 1) * n, base + (size_t)(cf.nt - 2) * n, n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        }
+        c->have_synth = true;
+        if (store) c->have_snaps = true;
+    }
+    return 0;
+}
+
+// (nt, ntr) column-major <-> time-major [t][ntr]
+void to_time_major(const float* f, int nt, int ntr, float* out) {
+    for (int r = 0; r < ntr; ++r)
+        for (int t = 0; t < nt; ++t) out[(size_t)t * ntr + r] = f[(size_t)r * nt + t];
+}
+void from_time_major(const float* tm, int nt, int ntr, float* out) {
+    for (int r = 0; r < ntr; ++r)
+        for (int t = 0; t < nt; ++t) out[(size_t)r * nt + t] = tm[(size_t)t * ntr + r];
+}
+
+int upload_traces(fwi_ctx* c, const float* host, int which /*0 src,1 obs,2 adj*/) {
+    const int nt = c->cfg.nt;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    size_t off = 0;
+    for (auto& s : c->shots) {
+        const int ntr = which == 0 ? (int)s.sx.size() : (int)s.rx.size();
+        std::vector<float> tm((size_t)ntr * nt);
+        to_time_major(host + off, nt, ntr, tm.data());
+        const size_t o = which == 0 ? s.o_src : (which == 1 ? s.o_obs : s.o_adj);
+        if (ntr) HIPCHK(hipMemcpy(c->traces.base + o, tm.data(), tm.size() * sizeof(float), hipMemcpyHostToDevice));
+        off += (size_t)ntr * nt;
+    }
+    return 0;
+}
+
+int download_traces(fwi_ctx* c, float* host, int which /*1 rec, 3 adj*/) {
+    const int nt = c->cfg.nt;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    size_t off = 0;
+    for (auto& s : c->shots) {
+        const int ntr = (int)s.rx.size();
+        std::vector<float> tm((size_t)ntr * nt);
+        const size_t o = which == 1 ? s.o_rec : s.o_adj;
+        if (ntr) HIPCHK(hipMemcpy(tm.data(), c->traces.base + o, tm.size() * sizeof(float), hipMemcpyDeviceToHost));
+        from_time_major(tm.data(), nt, ntr, host + off);
+        off += (size_t)ntr * nt;
+    }
+    return 0;
+}
+
+int ctx_init(fwi_ctx* c, const fwi_config* cfg, int nt_store) {
+    c->cfg = *cfg;
+    c->dn = cfg->dpml + 2 * fwi::ACC;
+    c->NY = cfg->ny + 2 * c->dn;
+    c->nt_store = nt_store;
+    c->vx = cfg->vx ? cfg->vx : 4;
+    c->rows = cfg->rows_per_task > 0 ? cfg->rows_per_task : 64;
+    cpml_table(cfg->dpml, cfg->dt, cfg->d, c->dn, c->fA, c->fB);
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fwi_last_error(void) { return g_err.c_str(); }
+
+int fwi_create(const fwi_config* cfg, fwi_ctx** out) {
+    if (!cfg || !out) return fail("null argument");
+    CHK(validate_config(*cfg));
+    if (cfg->nt % cfg->store_snap != 0) return fail("nt must be a multiple of store_snap");
+    fwi_ctx* c = new fwi_ctx();
+    int r = ctx_init(c, cfg, cfg->nt / cfg->store_snap);
+    if (r) { delete c; return r; }
+    *out = c;
+    return 0;
+}
+
+void fwi_destroy(fwi_ctx* ctx) { delete ctx; }
+
+int fwi_set_shots(fwi_ctx* c, int nshots, const fwi_shot* shots) {
+    if (!c || nshots < 1 || !shots) return fail("bad arguments to fwi_set_shots");
+    return setup_shots(c, nshots, shots);
+}
+
+int fwi_set_model(fwi_ctx* c, const float* vp) {
+    if (!c || !c->have_shots) return fail("fwi_set_model: set shots first");
+    const fwi_config& cf = c->cfg;
+    HIPCHK(hipMemcpyAsync(c->d_vp, vp, (size_t)cf.ny * cf.nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, cf.nx, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
+                                c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_model = true;
+    return 0;
+}
+
+int fwi_set_sources(fwi_ctx* c, const float* S) {
+    if (!c || !c->have_shots) return fail("fwi_set_sources: set shots first");
+    return upload_traces(c, S, 0);
+}
+
+int fwi_set_observed(fwi_ctx* c, const float* D) {
+    if (!c || !c->have_shots) return fail("fwi_set_observed: set shots first");
+    return upload_traces(c, D, 1);
+}
+
+int fwi_observed_from_synthetic(fwi_ctx* c) {
+    if (!c || !c->have_synth) return fail("no synthetic traces yet");
+    const size_t n = (size_t)c->nrec_total * c->cfg.nt * sizeof(float);
+    if (n) HIPCHK(hipMemcpyAsync(c->traces.base + c->shots[0].o_obs, c->traces.base + c->shots[0].o_rec, n,
+                                 hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_forward(fwi_ctx* c, int store_snapshots) {
+    if (!c || !c->have_model) return fail("fwi_forward: model not set");
+    return run(c, 0, store_snapshots != 0);
+}
+
+int fwi_get_synthetic(fwi_ctx* c, float* out) {
+    if (!c || !c->have_synth) return fail("no synthetic traces yet");
+    return download_traces(c, out, 1);
+}
+
+int fwi_misfit(fwi_ctx* c, double* total) {
+    if (!c || !c->have_synth) return fail("fwi_misfit: run fwi_forward first");
+    const int ns = (int)c->shots.size();
+    const Shot& s0 = c->shots[0];
+    HIPCHK(fwi::launch_misfit(c->traces.at<float>(s0.o_rec - s0.toff * sizeof(float)),
+                              c->traces.at<float>(s0.o_obs - s0.toff * sizeof(float)), c->d_recmap, c->nrec_total,
+                              c->d_resshots, ns, c->cfg.nt, c->cfg.dt,
+                              c->d_invs, c->traces.at<float>(s0.o_adj - s0.toff * sizeof(float)), c->d_partial,
+                              c->nblk_misfit, c->d_shot_misfit, c->stream));
+    if (total) {
+        std::vector<double> m(ns);
+        HIPCHK(hipMemcpyAsync(m.data(), c->d_shot_misfit, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double t = 0.0;
+        for (double v : m) t += v;
+        *total = t;
+    }
+    return 0;
+}
+
+int fwi_adjoint(fwi_ctx* c) {
+    if (!c || !c->have_snaps) return fail("fwi_adjoint: run fwi_forward with snapshots first");
+    HIPCHK(hipMemsetAsync(c->grads.base, 0, c->grads.size, c->stream));
+    return run(c, 1, false);
+}
+
+int fwi_reduce_gradient(fwi_ctx* c, double scale, const int* water_rows) {
+    if (!c || !c->have_shots) return fail("fwi_reduce_gradient: no shots");
+    const int* dw = nullptr;
+    if (water_rows) {
+        HIPCHK(hipMemcpyAsync(c->d_water, water_rows, c->cfg.nx * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        dw = c->d_water;
+    }
+    HIPCHK(fwi::launch_gradient_reduce(c->d_gradshots, (int)c->shots.size(), c->cfg.ny, c->cfg.nx, scale, dw,
+                                       c->d_grad, c->stream));
+    return 0;
+}
+
+int fwi_gradient(fwi_ctx* c, double scale, const int* water_rows, double* misfit_total) {
+    CHK(fwi_forward(c, 1));
+    CHK(fwi_misfit(c, misfit_total));
+    CHK(fwi_adjoint(c));
+    CHK(fwi_reduce_gradient(c, scale, water_rows));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_get_gradient(fwi_ctx* c, double* grad) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(grad, c->d_grad, (size_t)c->cfg.ny * c->cfg.nx * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fwi_get_kernels(fwi_ctx* c, float* out) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int ny = c->cfg.ny;
+    size_t off = 0;
+    for (auto& s : c->shots) {
+        std::vector<float> k((size_t)ny * s.Ws);
+        HIPCHK(hipMemcpy(k.data(), c->grads.base + s.o_grad, k.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int i = 0; i < s.W; ++i)
+            for (int j = 0; j < ny; ++j) out[off + (size_t)i * ny + j] = k[(size_t)j * s.Ws + i];
+        off += (size_t)ny * s.W;
+    }
+    return 0;
+}
+
+int fwi_get_adjoint_sources(fwi_ctx* c, float* out) {
+    if (!c) return fail("null context");
+    return download_traces(c, out, 3);
+}
+
+int fwi_get_shot_misfits(fwi_ctx* c, double* out) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_shot_misfit, c->shots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fwi_sync(fwi_ctx* c) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_set_timing(fwi_ctx* c, int enable) {
+    if (!c) return fail("null context");
+    c->timing = enable != 0;
+    return 0;
+}
+
+int fwi_get_timing(fwi_ctx* c, double* step_kernel_ms, long long* step_launches, int reset) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& e : c->ev_pending) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        c->step_ms += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    c->ev_pending.clear();
+    if (step_kernel_ms) *step_kernel_ms = c->step_ms;
+    if (step_launches) *step_launches = c->step_launches;
+    if (reset) { c->step_ms = 0.0; c->step_launches = 0; }
+    return 0;
+}
+
+long long fwi_device_bytes(fwi_ctx* c) {
+    if (!c) return 0;
+    return (long long)(c->state.size + c->stat.size + c->traces.size + c->snaps.size + c->grads.size + c->misc.size);
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in solver (solver_acoustic_SS.f90:9-11)
+
+int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int NumRec, const int* nxSou,
+                        const int* nySou, const int* nxRec, const int* nyRec, const float* S, const float* Sadj,
+                        float d, float dt, int ny, int nx, const float* ctmp, int store_snap, int nt_Store, int nt,
+                        float* kernel, float* Fields, float* Rec_point) {
+    fwi_solver_opts o = opts ? *opts : fwi_solver_opts{20, 1};
+    if (Rec_point && NumRec > 0) std::memset(Rec_point, 0, sizeof(float) * (size_t)nt * NumRec);   // :92
+    if (iprop == 1 && !Fields) return fail("iprop=1 needs Fields");
+    if (iprop == 2 && (!Fields || !kernel)) return fail("iprop=2 needs Fields and kernel");
+    if (NumSou < 0 || NumRec < 0) return fail("negative source/receiver count");
+    fwi_config cfg{};
+    cfg.ny = ny; cfg.nx = nx; cfg.d = d; cfg.dt = dt; cfg.nt = nt;
+    cfg.store_snap = store_snap > 0 ? store_snap : 1;
+    cfg.dpml = o.dpml; cfg.free_surface = o.free_surface; cfg.snap_kind = FWI_SNAP_F32;
+    CHK(validate_config(cfg));
+    const bool snaps = (iprop == 1 || iprop == 2);
+    if (snaps && nt_Store < 0) return fail("negative nt_Store");
+    fwi_ctx c;
+    CHK(ctx_init(&c, &cfg, snaps ? nt_Store : 0));
+    fwi_shot sh{};
+    sh.x0 = 0; sh.w = nx;
+    // iprop=2 injects at the receivers (solver_acoustic_SS.f90:114-118)
+    sh.nsrc = iprop == 2 ? 0 : NumSou; sh.src_x = nxSou; sh.src_y = nySou;
+    sh.nrec = NumRec; sh.rec_x = nxRec; sh.rec_y = nyRec;
+    CHK(setup_shots(&c, 1, &sh));
+    CHK(fwi_set_model(&c, ctmp));
+    const Shot& s = c.shots[0];
+    const size_t slot_elems = (size_t)ny * s.Ws;
+    if (iprop != 2) {
+        if (NumSou) CHK(upload_traces(&c, S, 0));
+        CHK(run(&c, 0, iprop == 1));
+        if (NumRec) CHK(download_traces(&c, Rec_point, 1));
+        if (iprop == 1) {
+            HIPCHK(hipStreamSynchronize(c.stream));
+            std::vector<float> slot(slot_elems);
+            const int last = std::min(nt_Store, nt / cfg.store_snap);
+            for (int m = 1; m <= last; ++m) {
+                HIPCHK(hipMemcpy(slot.data(), c.snaps.base + s.o_snap + (size_t)m * slot_elems * sizeof(float),
+                                 slot_elems * sizeof(float), hipMemcpyDeviceToHost));
+                float* F = Fields + (size_t)m * nx * ny;
+                for (int i = 0; i < nx; ++i)
+                    for (int j = 0; j < ny; ++j) F[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i];
+            }
+        }
+    } else {
+        if (NumRec) CHK(upload_traces(&c, Sadj, 2));
+        std::vector<float> slot(slot_elems, 0.0f);
+        for (int m = 0; m <= nt_Store; ++m) {
+            const float* F = Fields + (size_t)m * nx * ny;
+            for (int i = 0; i < nx; ++i)
+                for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i] = F[(size_t)i * ny + j];
+            HIPCHK(hipMemcpy(c.snaps.base + s.o_snap + (size_t)m * slot_elems * sizeof(float), slot.data(),
+                             slot_elems * sizeof(float), hipMemcpyHostToDevice));
+        }
+        std::fill(slot.begin(), slot.end(), 0.0f);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i] = kernel[(size_t)i * ny + j];
+        HIPCHK(hipMemcpy(c.grads.base + s.o_grad, slot.data(), slot_elems * sizeof(float), hipMemcpyHostToDevice));
+        c.have_snaps = true;
+        CHK(run(&c, 1, false));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        HIPCHK(hipMemcpy(slot.data(), c.grads.base + s.o_grad, slot_elems * sizeof(float), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) kernel[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i];
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+void solver_acoustic_ss_(const int* rank, const int* iprop, const int* NumSou, const int* NumRec, const int* nxSou,
+                         const int* nySou, const int* nxRec, const int* nyRec, const float* S, const float* Sadj,
+                         const float* d, const float* dt, const int* ny, const int* nx, const float* ctmp,
+                         const int* store_snap, const int* nt_Store, const int* nt, float* kernel, float* Fields,
+                         float* Rec_point) {
+    (void)rank;
+    fwi_solver_opts o{20, 1};
+    const int r = fwi_solver_acoustic(&o, *iprop, *NumSou, *NumRec, nxSou, nySou, nxRec, nyRec, S, Sadj, *d, *dt, *ny,
+                                      *nx, ctmp, *store_snap, *nt_Store, *nt, kernel, Fields, Rec_point);
+    if (r != 0) {
+        std::fprintf(stderr, "solver_acoustic_SS (libfwi_hip): %s\n", g_err.c_str());
+        std::abort();
+    }
+}
+
+void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySou, const int* nxRec, const int* nyRec,
+                            const float* S, const float* d, const float* dt, const int* ny, const int* nx,
+                            const float* ctmp, const int* nt, float* Rec_point) {
+    fwi_solver_opts o{50, 1};
+    const int r = fwi_solver_acoustic(&o, 3, 1, *NumRec, nxSou, nySou, nxRec, nyRec, S, nullptr, *d, *dt, *ny, *nx,
+                                      ctmp, 1, 0, *nt, nullptr, nullptr, Rec_point);
+    if (r != 0) {
+        std::fprintf(stderr, "solver_forward_simple (libfwi_hip): %s\n", g_err.c_str());
+        std::abort();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// Auxiliary kernels of the gradient path (device-side data descriptors and launchers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include <algorithm>
+
+namespace fwi {
+
+struct C2Shot {       // padded c2 of one shot
+    float* c2;
+    int NX, P, W, x0;
+};
+
+struct RecMap {       // global receiver index -> its trace
+    size_t toff;      // element offset of the shot's trace block
+    int r, nrec;
+};
+
+struct ResShot {      // one shot's trace block
+    size_t toff;
+    int roff, nrec;
+};
+
+struct GradShot {     // one shot's FP32 imaging kernel
+    const float* k;
+    int x0, W, Ws, pad;
+};
+
+hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
+                           float dt, float d, hipStream_t st);
+hipError_t launch_misfit(const float* synth, const float* obs, const RecMap* map, int nrec_total, const ResShot* shots,
+                         int nshots, int nt, float dt, float* invs, float* adj, double* partial, int nblk_x,
+                         double* shot_misfit, hipStream_t st);
+hipError_t launch_gradient_reduce(const GradShot* shots, int nshots, int ny, int nxm, double scale,
+                                  const int* water_rows, double* grad, hipStream_t st);
+
+}  // namespace fwi

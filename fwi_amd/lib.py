@@ -1,0 +1,287 @@
+"""ctypes binding of libfwi_hip.so (the C ABI declared in include/fwi.h).
+
+This is plumbing: every compute call goes through the HIP library.  There is
+no CPU fallback -- if the library is missing or fails to load, the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfwi_hip.so")
+
+SNAP_F32, SNAP_BF16, SNAP_F16 = 0, 1, 2
+
+_P = C.c_void_p
+_lib = None
+
+
+class FwiError(RuntimeError):
+    pass
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("dpml", C.c_int), ("free_surface", C.c_int)]
+
+
+class Config(C.Structure):
+    _fields_ = [("ny", C.c_int), ("nx", C.c_int), ("d", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
+                ("store_snap", C.c_int), ("dpml", C.c_int), ("free_surface", C.c_int), ("snap_kind", C.c_int),
+                ("vx", C.c_int), ("rows_per_task", C.c_int)]
+
+
+class Shot(C.Structure):
+    _fields_ = [("x0", C.c_int), ("w", C.c_int), ("nsrc", C.c_int), ("src_x", _P), ("src_y", _P),
+                ("nrec", C.c_int), ("rec_x", _P), ("rec_y", _P)]
+
+
+def lib():
+    """Load libfwi_hip.so (raises FwiError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FwiError(f"{LIB_PATH} is not built; run `python fwi_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        L.fwi_last_error.restype = C.c_char_p
+        L.fwi_device_bytes.restype = C.c_longlong
+        L.fwi_create.argtypes = [C.POINTER(Config), C.POINTER(_P)]
+        L.fwi_destroy.argtypes = [_P]
+        L.fwi_destroy.restype = None
+        for name in ("fwi_set_shots", "fwi_set_model", "fwi_set_sources", "fwi_set_observed",
+                     "fwi_observed_from_synthetic", "fwi_forward", "fwi_get_synthetic", "fwi_misfit", "fwi_adjoint",
+                     "fwi_reduce_gradient", "fwi_gradient", "fwi_get_gradient", "fwi_get_kernels",
+                     "fwi_get_adjoint_sources", "fwi_get_shot_misfits", "fwi_sync", "fwi_solver_acoustic",
+                     "fwi_set_timing", "fwi_get_timing"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().fwi_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise FwiError(f"{what}: {last_error()}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _f32f(a):
+    return np.asfortranarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def solver_acoustic_SS(iprop, nxSou, nySou, nxRec, nyRec, S, Sadj, d, dt, ctmp, store_snap, nt_Store, nt,
+                       kernel=None, Fields=None, Rec_point=None, dpml=20, free_surface=1):
+    """Mirror of the reference's ``solver_acoustic_SS`` (solver_acoustic_SS.f90:9-11).
+
+    Arrays follow the reference: 1-based positions, Fortran-ordered
+    ``S(nt,NumSou)``, ``Sadj(nt,NumRec)``, ``ctmp(ny,nx)``, ``kernel(ny,nx)``
+    (updated in place by iprop=2), ``Fields(ny, nx*(nt_Store+1))`` (written by
+    iprop=1, read by iprop=2) and ``Rec_point(nt,NumRec)`` (always zeroed,
+    written by forward runs).  Returns ``(Rec_point, Fields, kernel)``.
+    """
+    ny, nx = ctmp.shape
+    nxSou, nySou, nxRec, nyRec = _i32(nxSou), _i32(nySou), _i32(nxRec), _i32(nyRec)
+    nsou, nrec = len(nxSou), len(nxRec)
+    S = _f32f(S if S is not None else np.zeros((nt, max(nsou, 1))))
+    Sadj = _f32f(Sadj if Sadj is not None else np.zeros((nt, max(nrec, 1))))
+    ctmp = _f32f(ctmp)
+    if kernel is None:
+        kernel = np.zeros((ny, nx), np.float32, order="F")
+    if Fields is None and iprop in (1, 2):
+        Fields = np.zeros((ny, nx * (nt_Store + 1)), np.float32, order="F")
+    if Rec_point is None:
+        Rec_point = np.zeros((nt, max(nrec, 1)), np.float32, order="F")
+    for name, a in (("kernel", kernel), ("Fields", Fields), ("Rec_point", Rec_point)):
+        if a is not None and not (a.dtype == np.float32 and a.flags.f_contiguous):
+            raise ValueError(f"{name} must be a Fortran-ordered float32 array")
+    opts = SolverOpts(dpml, free_surface)
+    rc = lib().fwi_solver_acoustic(C.byref(opts), C.c_int(iprop), C.c_int(nsou), C.c_int(nrec), _ptr(nxSou),
+                                   _ptr(nySou), _ptr(nxRec), _ptr(nyRec), _ptr(S), _ptr(Sadj), C.c_float(d),
+                                   C.c_float(dt), C.c_int(ny), C.c_int(nx), _ptr(ctmp), C.c_int(store_snap),
+                                   C.c_int(nt_Store), C.c_int(nt), _ptr(kernel), _ptr(Fields), _ptr(Rec_point))
+    _check(rc, "fwi_solver_acoustic")
+    return Rec_point, Fields, kernel
+
+
+def solver_acoustic_ss_dropin(rank, iprop, nxSou, nySou, nxRec, nyRec, S, Sadj, d, dt, ctmp, store_snap, nt_Store,
+                              nt, kernel, Fields, Rec_point):
+    """Calls the exported Fortran-convention symbol ``solver_acoustic_ss_`` exactly as a
+    Fortran caller would (every argument by reference)."""
+    L = lib()
+    ny, nx = ctmp.shape
+    nxSou, nySou, nxRec, nyRec = _i32(nxSou), _i32(nySou), _i32(nxRec), _i32(nyRec)
+    r = lambda v: C.byref(C.c_int(int(v)))
+    f = lambda v: C.byref(C.c_float(float(v)))
+    L.solver_acoustic_ss_(r(rank), r(iprop), r(len(nxSou)), r(len(nxRec)), _ptr(nxSou), _ptr(nySou), _ptr(nxRec),
+                          _ptr(nyRec), _ptr(_f32f(S)), _ptr(_f32f(Sadj)), f(d), f(dt), r(ny), r(nx), _ptr(_f32f(ctmp)),
+                          r(store_snap), r(nt_Store), r(nt), _ptr(kernel), _ptr(Fields), _ptr(Rec_point))
+    return Rec_point, Fields, kernel
+
+
+def solver_forward_simple(nxSou, nySou, nxRec, nyRec, S, d, dt, ctmp, nt):
+    """Mirror of ``solver_forward_simple`` (solver_forward_simple.f90:4-5) via the exported symbol."""
+    L = lib()
+    ny, nx = ctmp.shape
+    nxRec, nyRec = _i32(nxRec), _i32(nyRec)
+    rec = np.zeros((nt, len(nxRec)), np.float32, order="F")
+    r = lambda v: C.byref(C.c_int(int(v)))
+    f = lambda v: C.byref(C.c_float(float(v)))
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    L.solver_forward_simple_(r(len(nxRec)), r(nxSou), r(nySou), _ptr(nxRec), _ptr(nyRec), _ptr(S), f(d), f(dt),
+                             r(ny), r(nx), _ptr(_f32f(ctmp)), r(nt), _ptr(rec))
+    return rec
+
+
+class Context:
+    """Batched shots resident on one GPU (the fwi_ctx C API)."""
+
+    def __init__(self, ny, nx, d, dt, nt, store_snap=10, dpml=20, free_surface=1, snap_kind=SNAP_F32, vx=0,
+                 rows_per_task=0):
+        self.cfg = Config(ny, nx, d, dt, nt, store_snap, dpml, free_surface, snap_kind, vx, rows_per_task)
+        h = _P()
+        _check(lib().fwi_create(C.byref(self.cfg), C.byref(h)), "fwi_create")
+        self.h = h
+        self.ny, self.nx, self.nt = ny, nx, nt
+        self.shots = []
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fwi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_shots(self, shots):
+        """shots: list of dicts with x0, w, src_x, src_y, rec_x, rec_y (1-based window coordinates)."""
+        arr = (Shot * len(shots))()
+        keep = []
+        for i, s in enumerate(shots):
+            sx, sy, rx, ry = (_i32(s[k]) for k in ("src_x", "src_y", "rec_x", "rec_y"))
+            keep += [sx, sy, rx, ry]
+            arr[i] = Shot(int(s["x0"]), int(s["w"]), len(sx), _ptr(sx), _ptr(sy), len(rx), _ptr(rx), _ptr(ry))
+        _check(lib().fwi_set_shots(self.h, len(shots), arr), "fwi_set_shots")
+        self.shots = [dict(x0=int(s["x0"]), w=int(s["w"]), nsrc=len(s["src_x"]), nrec=len(s["rec_x"])) for s in shots]
+
+    def set_model(self, vp):
+        vp = _f32f(vp)
+        assert vp.shape == (self.ny, self.nx)
+        _check(lib().fwi_set_model(self.h, _ptr(vp)), "fwi_set_model")
+
+    def _concat(self, per_shot, key):
+        parts = []
+        for s, a in zip(self.shots, per_shot):
+            a = _f32f(a)
+            assert a.shape == (self.nt, s[key]), (a.shape, s[key])
+            parts.append(a.ravel(order="F"))
+        return np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1, np.float32))
+
+    def set_sources(self, S_list):
+        _check(lib().fwi_set_sources(self.h, _ptr(self._concat(S_list, "nsrc"))), "fwi_set_sources")
+
+    def set_observed(self, D_list):
+        _check(lib().fwi_set_observed(self.h, _ptr(self._concat(D_list, "nrec"))), "fwi_set_observed")
+
+    def observed_from_synthetic(self):
+        _check(lib().fwi_observed_from_synthetic(self.h), "fwi_observed_from_synthetic")
+
+    def _split(self, flat, key, rows):
+        out, off = [], 0
+        for s in self.shots:
+            n = rows * s[key]
+            out.append(flat[off:off + n].reshape((rows, s[key]), order="F"))
+            off += n
+        return out
+
+    def forward(self, store_snapshots=False):
+        _check(lib().fwi_forward(self.h, int(store_snapshots)), "fwi_forward")
+
+    def synthetic(self):
+        n = sum(s["nrec"] for s in self.shots) * self.nt
+        buf = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_get_synthetic(self.h, _ptr(buf)), "fwi_get_synthetic")
+        return self._split(buf, "nrec", self.nt)
+
+    def adjoint_sources(self):
+        n = sum(s["nrec"] for s in self.shots) * self.nt
+        buf = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_get_adjoint_sources(self.h, _ptr(buf)), "fwi_get_adjoint_sources")
+        return self._split(buf, "nrec", self.nt)
+
+    def misfit(self):
+        m = C.c_double(0)
+        _check(lib().fwi_misfit(self.h, C.byref(m)), "fwi_misfit")
+        return m.value
+
+    def shot_misfits(self):
+        out = np.zeros(len(self.shots), np.float64)
+        _check(lib().fwi_get_shot_misfits(self.h, _ptr(out)), "fwi_get_shot_misfits")
+        return out
+
+    def adjoint(self):
+        _check(lib().fwi_adjoint(self.h), "fwi_adjoint")
+
+    def reduce_gradient(self, scale=1.0, water_rows=None):
+        w = _i32(water_rows) if water_rows is not None else None
+        _check(lib().fwi_reduce_gradient(self.h, C.c_double(scale), _ptr(w)), "fwi_reduce_gradient")
+
+    def gradient(self, scale=1.0, water_rows=None):
+        """forward -> misfit -> adjoint -> FP64 shot sum.  Returns the total misfit."""
+        w = _i32(water_rows) if water_rows is not None else None
+        m = C.c_double(0)
+        _check(lib().fwi_gradient(self.h, C.c_double(scale), _ptr(w), C.byref(m)), "fwi_gradient")
+        return m.value
+
+    def get_gradient(self):
+        g = np.zeros((self.ny, self.nx), np.float64, order="F")
+        _check(lib().fwi_get_gradient(self.h, _ptr(g)), "fwi_get_gradient")
+        return g
+
+    def kernels(self):
+        n = sum(s["w"] for s in self.shots) * self.ny
+        buf = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_get_kernels(self.h, _ptr(buf)), "fwi_get_kernels")
+        out, off = [], 0
+        for s in self.shots:
+            k = self.ny * s["w"]
+            out.append(buf[off:off + k].reshape((self.ny, s["w"]), order="F"))
+            off += k
+        return out
+
+    def sync(self):
+        _check(lib().fwi_sync(self.h), "fwi_sync")
+
+    def set_timing(self, enable=True):
+        _check(lib().fwi_set_timing(self.h, int(enable)), "fwi_set_timing")
+
+    def timing(self, reset=False):
+        """(step-kernel GPU milliseconds, step launches) accumulated since the last reset."""
+        ms, n = C.c_double(0), C.c_longlong(0)
+        _check(lib().fwi_get_timing(self.h, C.byref(ms), C.byref(n), int(reset)), "fwi_get_timing")
+        return ms.value, n.value
+
+    def device_bytes(self):
+        return int(lib().fwi_device_bytes(self.h))

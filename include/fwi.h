@@ -1,0 +1,171 @@
+/*
+ * fwi.h -- C ABI of libfwi_hip.so, the MI355X (gfx950) implementation of the
+ * forward/adjoint acoustic solver and FWI gradient path of ejimeneztejero/FWI.
+ *
+ * Two layers:
+ *
+ * 1. Drop-in symbols with the reference's Fortran calling convention
+ *    (implicit interface: every argument by reference, REAL(4)/INTEGER(4)
+ *    column-major arrays, lowercase name with a trailing underscore).  A
+ *    Fortran program that links this library instead of the reference's
+ *    solver objects runs unchanged.
+ *
+ * 2. A batched context API (own design) that keeps many shots resident on
+ *    one GPU: per-shot aperture windows, device-resident snapshots, misfit,
+ *    adjoint source, FP64 shot-summed gradient.  Integer status returns
+ *    (0 = success); fwi_last_error() describes the most recent failure.
+ *
+ * All pointers in this header are host pointers.
+ */
+#ifndef FWI_H
+#define FWI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in replacements                                                    */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces subroutine solver_acoustic_SS (src/solver_acoustic_SS.f90:9-11),
+ * called from src/calculate_misfit_gradient.f90:259-261 (iprop=1) and :315-317
+ * (iprop=2).  Same arguments and meaning:
+ *   iprop=1 forward; writes Rec_point(nt,NumRec) and snapshot slots
+ *           1..nt_Store of Fields(ny, nx*(nt_Store+1)).
+ *   iprop=2 adjoint from Sadj(nt,NumRec) injected at the receiver cells;
+ *           kernel(ny,nx) += imaging condition against Fields; Rec_point is
+ *           zeroed.
+ *   other   forward without snapshots.
+ * CPML thickness 20 cells, free surface on, 6th-order stencil -- the
+ * reference's hard-coded values (src/solver_acoustic_SS.f90:90-93).
+ * `rank` is accepted and ignored (the reference only uses it for a dead
+ * debug branch).  On an error the routine prints a message and aborts, as
+ * the reference's runtime does on allocation failure.
+ */
+void solver_acoustic_ss_(const int* rank, const int* iprop, const int* NumSou, const int* NumRec,
+                         const int* nxSou, const int* nySou, const int* nxRec, const int* nyRec,
+                         const float* S, const float* Sadj, const float* d, const float* dt,
+                         const int* ny, const int* nx, const float* ctmp, const int* store_snap,
+                         const int* nt_Store, const int* nt, float* kernel, float* Fields,
+                         float* Rec_point);
+
+/*
+ * Replaces subroutine solver_forward_simple (src/solver_forward_simple.f90:4-5),
+ * called from src/source_estimation.f90:86: forward modelling with a
+ * 50-cell CPML, one source injected at one cell, free surface on.
+ */
+void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySou, const int* nxRec,
+                            const int* nyRec, const float* S, const float* d, const float* dt,
+                            const int* ny, const int* nx, const float* ctmp, const int* nt,
+                            float* Rec_point);
+
+/* Solver options that the reference hard-codes (src/solver_acoustic_SS.f90:90-93). */
+typedef struct fwi_solver_opts {
+    int dpml;          /* CPML thickness in cells; reference: 20 (50 in solver_forward_simple) */
+    int free_surface;  /* 1: rows above the model held at zero (reference); 0: CPML on top too */
+} fwi_solver_opts;
+
+/*
+ * solver_acoustic_SS with explicit options and a status return.  Arguments
+ * as in solver_acoustic_ss_, passed by value where scalar.  Fields may be
+ * NULL for iprop other than 1 and 2; kernel may be NULL unless iprop=2.
+ */
+int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int NumRec,
+                        const int* nxSou, const int* nySou, const int* nxRec, const int* nyRec,
+                        const float* S, const float* Sadj, float d, float dt, int ny, int nx,
+                        const float* ctmp, int store_snap, int nt_Store, int nt, float* kernel,
+                        float* Fields, float* Rec_point);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Batched context API                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fwi_ctx fwi_ctx;
+
+enum { FWI_SNAP_F32 = 0, FWI_SNAP_BF16 = 1, FWI_SNAP_F16 = 2 };
+
+typedef struct fwi_config {
+    int ny, nx;            /* model rows (depth) and columns */
+    float d, dt;           /* grid spacing [m], time step [s] */
+    int nt;                /* time samples; must be a multiple of store_snap */
+    int store_snap;        /* snapshot interval in steps (reference: 10) */
+    int dpml;              /* CPML thickness (reference: 20) */
+    int free_surface;      /* as in fwi_solver_opts */
+    int snap_kind;         /* FWI_SNAP_F32 / _BF16 / _F16 */
+    int vx;                /* kernel columns per lane: 1, 2 or 4 (0 = default) */
+    int rows_per_task;     /* rows per work unit (0 = default) */
+} fwi_config;
+
+/* One shot: an aperture window of the model plus its acquisition, positions
+ * 1-based in window coordinates (column 1 is model column x0+1). */
+typedef struct fwi_shot {
+    int x0;                /* window start, 0-based model column */
+    int w;                 /* window width in columns */
+    int nsrc;
+    const int* src_x;      /* nsrc columns */
+    const int* src_y;      /* nsrc rows */
+    int nrec;
+    const int* rec_x;
+    const int* rec_y;
+} fwi_shot;
+
+int fwi_create(const fwi_config* cfg, fwi_ctx** out);
+void fwi_destroy(fwi_ctx* ctx);
+const char* fwi_last_error(void);
+
+/* Acquisition: allocates all device state for the shots. */
+int fwi_set_shots(fwi_ctx* ctx, int nshots, const fwi_shot* shots);
+/* Velocity model vp(ny, nx), column-major, m/s. */
+int fwi_set_model(fwi_ctx* ctx, const float* vp);
+/* Source wavelets: shot after shot, each S(nt, nsrc_s) column-major. */
+int fwi_set_sources(fwi_ctx* ctx, const float* S);
+/* Observed data: shot after shot, each D(nt, nrec_s) column-major. */
+int fwi_set_observed(fwi_ctx* ctx, const float* D);
+/* Use the synthetic traces of the last forward run as observed data. */
+int fwi_observed_from_synthetic(fwi_ctx* ctx);
+
+/* Forward modelling of every shot; store_snapshots=1 keeps the imaging
+ * snapshots on the device for a following adjoint run. */
+int fwi_forward(fwi_ctx* ctx, int store_snapshots);
+/* Synthetic traces of the last forward run, shot after shot, (nt, nrec_s) each. */
+int fwi_get_synthetic(fwi_ctx* ctx, float* out);
+
+/* Misfit and adjoint sources on the device (Misfit.f90:28-70 with
+ * Misfit_L2.f90).  Per-shot misfits are accumulated in FP64. */
+int fwi_misfit(fwi_ctx* ctx, double* misfit_total);
+/* Adjoint run of every shot with the current adjoint sources, applying the
+ * imaging condition into per-shot FP32 kernels. */
+int fwi_adjoint(fwi_ctx* ctx);
+/* Shot-summed gradient in FP64: grad = scale * sum_s kernel_s placed at each
+ * window, rows above water_rows[x] (1-based count per column) zeroed
+ * (calculate_misfit_gradient.f90:339-351).  water_rows may be NULL. */
+int fwi_reduce_gradient(fwi_ctx* ctx, double scale, const int* water_rows);
+/* forward(+snapshots) -> misfit -> adjoint -> reduce, all on the device. */
+int fwi_gradient(fwi_ctx* ctx, double scale, const int* water_rows, double* misfit_total);
+/* Copy the last reduced gradient, (ny, nx) column-major FP64. */
+int fwi_get_gradient(fwi_ctx* ctx, double* grad);
+/* Copy per-shot imaging kernels (FP32, (ny, w_s) column-major each, shot after shot). */
+int fwi_get_kernels(fwi_ctx* ctx, float* out);
+/* Copy the adjoint sources (nt, nrec_s) per shot. */
+int fwi_get_adjoint_sources(fwi_ctx* ctx, float* out);
+/* Per-shot misfits of the last fwi_misfit call (FP64). */
+int fwi_get_shot_misfits(fwi_ctx* ctx, double* out);
+
+/* Step-kernel timing: when enabled, every sweep records HIP events on the
+ * context's stream around its step launches.  fwi_get_timing returns the
+ * accumulated GPU time of those spans and the number of step launches in
+ * them (reset=1 clears the totals). */
+int fwi_set_timing(fwi_ctx* ctx, int enable);
+int fwi_get_timing(fwi_ctx* ctx, double* step_kernel_ms, long long* step_launches, int reset);
+
+/* Synchronise the context's stream. */
+int fwi_sync(fwi_ctx* ctx);
+/* Device bytes held by the context. */
+long long fwi_device_bytes(fwi_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FWI_H */

@@ -1,0 +1,65 @@
+#!/bin/bash
+# Builds the reference's own Fortran solver path into oracle/_ref/ as shared
+# libraries that the tests load with ctypes.  This is test infrastructure: the
+# product never links or loads anything built here.
+#
+# Sources are compiled where they lie under the reference tree (default
+# /root/reference/src); no reference source is copied into this repository.
+# Outputs (all git-ignored):
+#   _ref/libfwiref_fs1_r4.so   unmodified sources, REAL(4)          (the parity oracle)
+#   _ref/libfwiref_fs1_r8.so   unmodified sources, -fdefault-real-8 (FP64 build of the same scheme)
+#   _ref/libfwiref_fs0_r4.so   free_surface=0 variant, REAL(4)
+#   _ref/libfwiref_fs0_r8.so   free_surface=0 variant, REAL(8)
+# The free_surface=0 variant changes the one hard-coded assignment
+# `free_surface=1` (solver_acoustic_SS.f90:93) to 0 in a temporary copy made
+# outside the repository; the copy is deleted when the script exits.
+# solver_forward_simple.f90 is compiled at -O0: it reads an uninitialised
+# `iprop` (solver_forward_simple.f90:10,97,105) and flang -O2/-O3 reduce the
+# routine to an empty body.
+set -euo pipefail
+REF_SRC=${FWI_REFERENCE_SRC:-/root/reference/src}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FL=${FLANG:-/opt/rocm/lib/llvm/bin/flang}
+
+if [ ! -f "$REF_SRC/solver_acoustic_SS.f90" ]; then
+    echo "build_ref: reference sources not found under $REF_SRC; nothing built"
+    exit 0
+fi
+if [ ! -x "$FL" ]; then
+    echo "build_ref: flang not found at $FL; nothing built"
+    exit 0
+fi
+
+TMP=$(mktemp -d)
+trap 'rm -rf "$TMP"' EXIT
+mkdir -p "$OUT"
+
+FILES="solver_space_derivs solver_PML_coeffs solver_PML_matrices solver_FD_coeffs Misfit_L2"
+
+# free_surface=0 variant of the solver (exactly one line may change)
+sed 's/^\([[:space:]]*\)free_surface=1/\1free_surface=0/' \
+    "$REF_SRC/solver_acoustic_SS.f90" > "$TMP/solver_acoustic_SS_fs0.f90"
+nchg=$(diff "$REF_SRC/solver_acoustic_SS.f90" "$TMP/solver_acoustic_SS_fs0.f90" | grep -c '^>' || true)
+if [ "$nchg" != "1" ]; then
+    echo "build_ref: expected exactly one changed line for the free_surface=0 variant, got $nchg" >&2
+    exit 1
+fi
+
+build_variant() {
+    local name=$1 solver_src=$2 realflag=$3
+    local dir="$TMP/$name"
+    mkdir -p "$dir"
+    $FL -O3 -fPIC $realflag -c "$solver_src" -o "$dir/solver_acoustic_SS.o"
+    for f in $FILES; do
+        $FL -O3 -fPIC $realflag -c "$REF_SRC/$f.f90" -o "$dir/$f.o"
+    done
+    $FL -O0 -fPIC $realflag -c "$REF_SRC/solver_forward_simple.f90" -o "$dir/solver_forward_simple.o"
+    $FL -shared -fPIC -Wl,-Bsymbolic -o "$OUT/libfwiref_$name.so" "$dir"/*.o
+    echo "build_ref: built $OUT/libfwiref_$name.so"
+}
+
+build_variant fs1_r4 "$REF_SRC/solver_acoustic_SS.f90" ""
+build_variant fs1_r8 "$REF_SRC/solver_acoustic_SS.f90" "-fdefault-real-8"
+build_variant fs0_r4 "$TMP/solver_acoustic_SS_fs0.f90" ""
+build_variant fs0_r8 "$TMP/solver_acoustic_SS_fs0.f90" "-fdefault-real-8"

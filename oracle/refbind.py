@@ -1,0 +1,183 @@
+"""ctypes bindings for the two CPU checkers used by the tests.
+
+* ``oracle/liboracle.so`` -- the C++ restatement (float and double).
+* ``oracle/_ref/libfwiref_*.so`` -- the reference's own Fortran routines,
+  compiled from its sources by ``oracle/build_ref.sh`` (present only where the
+  reference tree and flang are; tests that need it skip otherwise).
+
+Both take the reference's argument conventions: 1-based model positions and
+Fortran column-major arrays.  Everything here is test infrastructure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))  # repository root
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_P = C.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _iref(v):
+    return C.byref(C.c_int(int(v)))
+
+
+class OracleOpts(C.Structure):
+    _fields_ = [("dpml", C.c_int), ("free_surface", C.c_int)]
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise FileNotFoundError(f"{ORACLE_SO} not built (run `make -C oracle`)")
+        _oracle = C.CDLL(ORACLE_SO)
+    return _oracle
+
+
+_refs: dict = {}
+
+
+def ref_available(variant="fs1_r4"):
+    return os.path.exists(os.path.join(REF_DIR, f"libfwiref_{variant}.so"))
+
+
+def ref(variant="fs1_r4"):
+    if variant not in _refs:
+        path = os.path.join(REF_DIR, f"libfwiref_{variant}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        _refs[variant] = C.CDLL(path)
+    return _refs[variant]
+
+
+def _f(a, dt):
+    return np.asfortranarray(a, dtype=dt)
+
+
+class SolverResult:
+    def __init__(self, rec, fields, kernel, status=0):
+        self.rec, self.fields, self.kernel, self.status = rec, fields, kernel, status
+
+
+def _prep(iprop, nxsou, nysou, nxrec, nyrec, S, Sadj, ctmp, nt, store_snap, nt_store, dt_np,
+          kernel=None, fields=None):
+    ny, nx = ctmp.shape
+    nxsou = np.ascontiguousarray(nxsou, dtype=np.int32)
+    nysou = np.ascontiguousarray(nysou, dtype=np.int32)
+    nxrec = np.ascontiguousarray(nxrec, dtype=np.int32)
+    nyrec = np.ascontiguousarray(nyrec, dtype=np.int32)
+    nsou, nrec = len(nxsou), len(nxrec)
+    S = _f(S if S is not None else np.zeros((nt, max(nsou, 1))), dt_np)
+    Sadj = _f(Sadj if Sadj is not None else np.zeros((nt, max(nrec, 1))), dt_np)
+    ctmp = _f(ctmp, dt_np)
+    kernel = _f(kernel if kernel is not None else np.zeros((ny, nx)), dt_np).copy(order="F")
+    if fields is None:
+        fields = np.zeros((ny, nx * (nt_store + 1)), dtype=dt_np, order="F")
+    else:
+        fields = _f(fields, dt_np).copy(order="F")
+    rec = np.zeros((nt, max(nrec, 1)), dtype=dt_np, order="F")
+    return nxsou, nysou, nxrec, nyrec, nsou, nrec, S, Sadj, ctmp, kernel, fields, rec
+
+
+def ref_solver(variant, iprop, nxsou, nysou, nxrec, nyrec, S, Sadj, d, dt, ctmp, store_snap, nt_store, nt,
+               kernel=None, fields=None):
+    """Call the reference ``solver_acoustic_SS`` (solver_acoustic_SS.f90:9)."""
+    lib = ref(variant)
+    dt_np = np.float64 if variant.endswith("r8") else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    ny, nx = ctmp.shape
+    (nxsou, nysou, nxrec, nyrec, nsou, nrec, S, Sadj, ctmp, kernel, fields, rec) = _prep(
+        iprop, nxsou, nysou, nxrec, nyrec, S, Sadj, ctmp, nt, store_snap, nt_store, dt_np, kernel, fields)
+    lib.solver_acoustic_ss_(_iref(0), _iref(iprop), _iref(nsou), _iref(nrec), _ptr(nxsou), _ptr(nysou),
+                            _ptr(nxrec), _ptr(nyrec), _ptr(S), _ptr(Sadj), C.byref(ct(d)), C.byref(ct(dt)),
+                            _iref(ny), _iref(nx), _ptr(ctmp), _iref(store_snap), _iref(nt_store), _iref(nt),
+                            _ptr(kernel), _ptr(fields), _ptr(rec))
+    return SolverResult(rec, fields, kernel)
+
+
+def oracle_solver(precision, iprop, nxsou, nysou, nxrec, nyrec, S, Sadj, d, dt, ctmp, store_snap, nt_store, nt,
+                  kernel=None, fields=None, dpml=20, free_surface=1):
+    """Call the C++ restatement of ``solver_acoustic_SS``; precision 'f32' or 'f64'."""
+    lib = oracle()
+    dt_np = np.float64 if precision == "f64" else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    ny, nx = ctmp.shape
+    (nxsou, nysou, nxrec, nyrec, nsou, nrec, S, Sadj, ctmp, kernel, fields, rec) = _prep(
+        iprop, nxsou, nysou, nxrec, nyrec, S, Sadj, ctmp, nt, store_snap, nt_store, dt_np, kernel, fields)
+    fn = getattr(lib, f"oracle_solver_{precision}")
+    fn.restype = C.c_int
+    opts = OracleOpts(dpml, free_surface)
+    st = fn(C.byref(opts), C.c_int(iprop), C.c_int(nsou), C.c_int(nrec), _ptr(nxsou), _ptr(nysou), _ptr(nxrec),
+            _ptr(nyrec), _ptr(S), _ptr(Sadj), ct(d), ct(dt), C.c_int(ny), C.c_int(nx), _ptr(ctmp),
+            C.c_int(store_snap), C.c_int(nt_store), C.c_int(nt), _ptr(kernel), _ptr(fields), _ptr(rec))
+    return SolverResult(rec, fields, kernel, st)
+
+
+def ref_forward_simple(nxsou, nysou, nxrec, nyrec, S, d, dt, ctmp, nt, variant="fs1_r4"):
+    """Call the reference ``solver_forward_simple`` (solver_forward_simple.f90:4)."""
+    lib = ref(variant)
+    ny, nx = ctmp.shape
+    nxrec = np.ascontiguousarray(nxrec, dtype=np.int32)
+    nyrec = np.ascontiguousarray(nyrec, dtype=np.int32)
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    ctmp = _f(ctmp, np.float32)
+    rec = np.zeros((nt, len(nxrec)), dtype=np.float32, order="F")
+    lib.solver_forward_simple_(_iref(len(nxrec)), _iref(nxsou), _iref(nysou), _ptr(nxrec), _ptr(nyrec), _ptr(S),
+                               C.byref(C.c_float(d)), C.byref(C.c_float(dt)), _iref(ny), _iref(nx), _ptr(ctmp),
+                               _iref(nt), _ptr(rec))
+    return rec
+
+
+def oracle_forward_simple(nxsou, nysou, nxrec, nyrec, S, d, dt, ctmp, nt):
+    lib = oracle()
+    ny, nx = ctmp.shape
+    nxrec = np.ascontiguousarray(nxrec, dtype=np.int32)
+    nyrec = np.ascontiguousarray(nyrec, dtype=np.int32)
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    ctmp = _f(ctmp, np.float32)
+    rec = np.zeros((nt, len(nxrec)), dtype=np.float32, order="F")
+    lib.oracle_forward_simple_f32.restype = C.c_int
+    st = lib.oracle_forward_simple_f32(C.c_int(len(nxrec)), C.c_int(nxsou), C.c_int(nysou), _ptr(nxrec),
+                                       _ptr(nyrec), _ptr(S), C.c_float(d), C.c_float(dt), C.c_int(ny), C.c_int(nx),
+                                       _ptr(ctmp), C.c_int(nt), _ptr(rec))
+    assert st == 0
+    return rec
+
+
+def ref_misfit_l2(synth, obs, dt, variant="fs1_r4"):
+    """Reference ``Misfit_L2`` (Misfit_L2.f90:1) on already-normalised traces."""
+    lib = ref(variant)
+    dt_np = np.float64 if variant.endswith("r8") else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    synth, obs = _f(synth, dt_np), _f(obs, dt_np)
+    nt, nrec = synth.shape
+    adj = np.zeros((nt, nrec), dtype=dt_np, order="F")
+    mis = ct(0)
+    lib.misfit_l2_(_ptr(synth), _ptr(obs), _iref(nrec), _iref(nt), C.byref(ct(dt)), C.byref(mis), _ptr(adj))
+    return float(mis.value), adj
+
+
+def oracle_misfit(synth, obs, dt, precision="f32"):
+    """Restated ``Misfit_function`` with misfit_type=1 (Misfit.f90:1-77, Misfit_L2.f90)."""
+    lib = oracle()
+    dt_np = np.float64 if precision == "f64" else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    synth, obs = _f(synth, dt_np), _f(obs, dt_np)
+    nt, nrec = synth.shape
+    adj = np.zeros((nt, nrec), dtype=dt_np, order="F")
+    mis = ct(0)
+    getattr(lib, f"oracle_misfit_l2_{precision}")(_ptr(synth), _ptr(obs), C.c_int(nrec), C.c_int(nt), ct(dt),
+                                                 C.byref(mis), _ptr(adj))
+    return float(mis.value), adj

@@ -1,0 +1,136 @@
+"""Writes the golden vectors under tests/golden/ by running the reference's own
+Fortran routines, compiled from its sources by oracle/build_ref.sh into
+oracle/_ref/ (this needs the reference tree and flang, i.e. this container).
+
+Fixtures (numpy .npz, no pickled objects):
+  tiny_fs{0,1}_r{4,8}.npz  40x60 model, nt=200, 2 sources, 7 receivers:
+                           forward seismograms, all snapshot slots (REAL(4)
+                           only), misfit and adjoint source, imaging kernel
+  config1_fs0_r4.npz       config-1 single-shot gradient path (150x400,
+                           nt=2000): imaging kernel, decimated seismograms,
+                           misfit, SHA-256 of the full seismogram and snapshots
+  pml_d{20,50}_r{4,8}.npz  CPML profiles and corner matrices (pml_coeffs_)
+  fd_r{4,8}.npz            finite_difference_coef_ tables (num 1,2; acc 1,3,4,5)
+
+The misfit step uses the reference Misfit_L2 (Misfit_L2.f90) on traces
+normalised as Misfit_function does (Misfit.f90:28-61); Misfit_function itself
+cannot be compiled unmodified here (its module needs an `iargc`
+declaration), so the normalisation (max-abs per trace, skipped for all-zero
+traces) is applied in numpy with REAL(4)/REAL(8) division.
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import refbind as rb  # noqa: E402
+from fwi_amd import synth  # noqa: E402
+
+
+def normalise(synth_tr, obs_tr, dt_np):
+    """Misfit.f90:31-61 on (nt, nrec) traces."""
+    DS = np.zeros_like(synth_tr, dtype=dt_np)
+    DT = np.zeros_like(obs_tr, dtype=dt_np)
+    for j in range(synth_tr.shape[1]):
+        if np.any(obs_tr[:, j] != 0):
+            mS = np.abs(synth_tr[:, j]).max()
+            mT = np.abs(obs_tr[:, j]).max()
+            if mS != 0 and mT != 0:
+                DS[:, j] = synth_tr[:, j] / mS
+                DT[:, j] = obs_tr[:, j] / mT
+    return np.asfortranarray(DS), np.asfortranarray(DT)
+
+
+def gradient_path(variant, c_true, c_init, S, nxs, nys, nxr, nyr, d, dt, nt, ss):
+    dt_np = np.float64 if variant.endswith("r8") else np.float32
+    nts = nt // ss
+    obs = rb.ref_solver(variant, 3, nxs, nys, nxr, nyr, S, None, d, dt, c_true, ss, nts, nt).rec
+    F = rb.ref_solver(variant, 1, nxs, nys, nxr, nyr, S, None, d, dt, c_init, ss, nts, nt)
+    DS, DT = normalise(F.rec.astype(dt_np), obs.astype(dt_np), dt_np)
+    mis, adj = rb.ref_misfit_l2(DS, DT, dt, variant)
+    A = rb.ref_solver(variant, 2, nxs, nys, nxr, nyr, S, adj, d, dt, c_init, ss, nts, nt, fields=F.fields)
+    return obs, F, mis, adj, A
+
+
+def tiny_inputs():
+    c_true, S, nxs, nys, nxr, nyr = synth.small_case(0, 40, 60, 200)
+    from scipy.ndimage import gaussian_filter
+    c_init = gaussian_filter(c_true, 3.0, mode="nearest").astype(np.float32)
+    return c_true, c_init, S, nxs, nys, nxr, nyr
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes(order="F")).hexdigest()
+
+
+def main():
+    if not rb.ref_available("fs1_r4"):
+        raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
+    d, dt, nt, ss = 25.0, 0.002, 200, 10
+    c_true, c_init, S, nxs, nys, nxr, nyr = tiny_inputs()
+    for fs in (0, 1):
+        for r in ("r4", "r8"):
+            v = f"fs{fs}_{r}"
+            dt_np = np.float64 if r == "r8" else np.float32
+            obs, F, mis, adj, A = gradient_path(v, c_true, c_init, S.astype(dt_np), nxs, nys, nxr, nyr,
+                                                dt_np(d), dt_np(dt), nt, ss)
+            out = dict(c_true=c_true, c_init=c_init, S=S, nxs=nxs, nys=nys, nxr=nxr, nyr=nyr, d=d, dt=dt, nt=nt,
+                       ss=ss, obs=obs, rec=F.rec, misfit=mis, adj=adj, kernel=A.kernel)
+            if r == "r4":
+                out["fields"] = F.fields
+            np.savez_compressed(os.path.join(HERE, f"tiny_{v}.npz"), **out)
+            print("wrote", v, "misfit", mis)
+
+    acq = synth.config1(free_surface=0)
+    sh = acq.shots[0]
+    f32 = np.float32
+    obs, F, mis, adj, A = gradient_path("fs0_r4", acq.vp_true, acq.vp_init, acq.wavelets[0], sh["src_x"],
+                                        sh["src_y"], sh["rec_x"], sh["rec_y"], f32(acq.h), f32(acq.dt), acq.nt,
+                                        acq.store_snap)
+    np.savez_compressed(os.path.join(HERE, "config1_fs0_r4.npz"), kernel=A.kernel, rec_dec=F.rec[::10, ::10],
+                        obs_dec=obs[::10, ::10], misfit=mis, adj_dec=adj[::10, ::10],
+                        rec_sha=sha(F.rec), obs_sha=sha(obs), adj_sha=sha(adj), fields_sha=sha(F.fields))
+    print("wrote config1 misfit", mis)
+
+    for dpml in (20, 50):
+        for r in ("r4", "r8"):
+            lib = rb.ref(f"fs1_{r}")
+            dt_np = np.float64 if r == "r8" else np.float32
+            ct = C.c_double if r == "r8" else C.c_float
+            dn = dpml + 6
+            fA = np.zeros(dn, dt_np)
+            fB = np.zeros(dn, dt_np)
+            mats = [np.zeros((dn, dn), dt_np, order="F") for _ in range(8)]
+            ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+            lib.pml_coeffs_(C.byref(C.c_int(3)), C.byref(C.c_int(dpml)), C.byref(ct(0.002)), C.byref(ct(25.0)),
+                            C.byref(C.c_int(dn)), ptr(fA), ptr(fB), *[ptr(m) for m in mats])
+            names = ["ldA", "rdA", "luA", "ruA", "ldB", "rdB", "luB", "ruB"]
+            np.savez_compressed(os.path.join(HERE, f"pml_d{dpml}_{r}.npz"), fA=fA, fB=fB,
+                                **dict(zip(names, mats)), dt=0.002, d=25.0)
+    for r in ("r4", "r8"):
+        lib = rb.ref(f"fs1_{r}")
+        dt_np = np.float64 if r == "r8" else np.float32
+        tab = {}
+        for num in (1, 2):
+            for acc in (1, 3, 4, 5):
+                coef = np.zeros(acc + 1, dt_np)
+                lib.finite_difference_coef_(C.byref(C.c_int(num)), C.byref(C.c_int(acc)),
+                                            coef.ctypes.data_as(C.c_void_p))
+                tab[f"n{num}_a{acc}"] = coef
+        np.savez_compressed(os.path.join(HERE, f"fd_{r}.npz"), **tab)
+    print("done")
+
+
+if __name__ == "__main__":
+    main()

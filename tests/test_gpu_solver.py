@@ -1,0 +1,175 @@
+"""GPU parity of the drop-in solver (HIP library) against the CPU oracle and,
+where it is built, the reference's own Fortran (oracle/_ref).
+
+Bar: bit-exact (np.array_equal, i.e. equal up to the sign of zero) for the
+seismograms, snapshots and imaging kernel of one solver call -- the kernels
+reproduce the reference's FP32 operation order without fused multiply-adds.
+"""
+import numpy as np
+import pytest
+
+import fwi_amd
+import refbind as rb
+from fwi_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed=0, ny=40, nx=60, nt=200):
+    c, S, nxs, nys, nxr, nyr = synth.small_case(seed, ny, nx, nt)
+    return c, S, nxs, nys, nxr, nyr, np.float32(25.0), np.float32(0.002)
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+@pytest.mark.parametrize("fs", [1, 0])
+def test_forward_adjoint_bitwise_vs_oracle(fs):
+    c, S, nxs, nys, nxr, nyr, d, dt = _case()
+    nt, ss = S.shape[0], 10
+    nts = nt // ss
+    rec, F, _ = fwi_amd.solver_acoustic_SS(1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nts, nt, free_surface=fs)
+    O = rb.oracle_solver("f32", 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nts, nt, free_surface=fs)
+    assert O.status == 0
+    assert np.abs(O.rec).max() > 0
+    assert np.array_equal(rec, O.rec), f"rec max diff {np.abs(rec - O.rec).max()}"
+    assert np.array_equal(F, O.fields), f"fields max diff {np.abs(F - O.fields).max()}"
+    adj = -O.rec[::-1].copy()
+    k0 = np.asfortranarray(np.random.default_rng(1).standard_normal(c.shape).astype(np.float32))
+    rec2, _, ker = fwi_amd.solver_acoustic_SS(2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nts, nt,
+                                              kernel=k0.copy(order="F"), Fields=O.fields.copy(order="F"),
+                                              free_surface=fs)
+    O2 = rb.oracle_solver("f32", 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nts, nt, kernel=k0,
+                          fields=O.fields, free_surface=fs)
+    assert np.abs(rec2).max() == 0
+    assert np.array_equal(ker, O2.kernel), f"kernel max diff {np.abs(ker - O2.kernel).max()}"
+
+
+@pytest.mark.skipif(not rb.ref_available("fs1_r4"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("fs", [1, 0])
+def test_dropin_symbol_bitwise_vs_reference(fs):
+    """The exported Fortran-convention symbol against the compiled reference routine."""
+    c, S, nxs, nys, nxr, nyr, d, dt = _case(seed=3, ny=48, nx=70, nt=300)
+    nt, ss = S.shape[0], 10
+    nts = nt // ss
+    R = rb.ref_solver(f"fs{fs}_r4", 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nts, nt)
+    if fs == 1:
+        F = np.zeros((c.shape[0], c.shape[1] * (nts + 1)), np.float32, order="F")
+        rec = np.ones((nt, len(nxr)), np.float32, order="F")
+        ker = np.zeros(c.shape, np.float32, order="F")
+        fwi_amd.solver_acoustic_ss_dropin(0, 1, nxs, nys, nxr, nyr, S, np.zeros((nt, len(nxr))), d, dt, c, ss, nts,
+                                          nt, ker, F, rec)
+    else:
+        rec, F, _ = fwi_amd.solver_acoustic_SS(1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nts, nt, free_surface=0)
+    assert np.array_equal(rec, R.rec), f"rec max diff {np.abs(rec - R.rec).max()}"
+    assert np.array_equal(F, R.fields)
+    adj = -R.rec[::-1].copy()
+    R2 = rb.ref_solver(f"fs{fs}_r4", 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nts, nt, fields=R.fields)
+    _, _, ker2 = fwi_amd.solver_acoustic_SS(2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nts, nt,
+                                            Fields=R.fields.copy(order="F"), free_surface=fs)
+    assert np.array_equal(ker2, R2.kernel), f"kernel max diff {np.abs(ker2 - R2.kernel).max()}"
+
+
+@pytest.mark.parametrize("dpml", [5, 20, 50])
+def test_cpml_thickness(dpml):
+    c, S, nxs, nys, nxr, nyr, d, dt = _case(seed=2)
+    nt = S.shape[0]
+    rec, _, _ = fwi_amd.solver_acoustic_SS(3, nxs, nys, nxr, nyr, S, None, d, dt, c, 1, 0, nt, dpml=dpml,
+                                           free_surface=0)
+    O = rb.oracle_solver("f32", 3, nxs, nys, nxr, nyr, S, None, d, dt, c, 1, 0, nt, dpml=dpml, free_surface=0)
+    assert np.array_equal(rec, O.rec)
+
+
+def test_forward_simple_vs_oracle():
+    c, S, nxs, nys, nxr, nyr, d, dt = _case(seed=4)
+    nt = S.shape[0]
+    rec = fwi_amd.solver_forward_simple(int(nxs[0]), int(nys[0]), nxr, nyr, S[:, 0], d, dt, c, nt)
+    O = rb.oracle_forward_simple(int(nxs[0]), int(nys[0]), nxr, nyr, S[:, 0], d, dt, c, nt)
+    assert np.abs(O).max() > 0
+    assert np.array_equal(rec, O)
+    if rb.ref_available("fs1_r4"):
+        R = rb.ref_forward_simple(int(nxs[0]), int(nys[0]), nxr, nyr, S[:, 0], d, dt, c, nt)
+        assert np.array_equal(rec, R)
+
+
+def test_duplicates_and_pml_positions():
+    """Sources/receivers sharing a cell (last one wins, Q9), a source inside the CPML
+    frame (ignored by the update) and a receiver in the frame (sampled)."""
+    c, S, _, _, _, _, d, dt = _case(seed=5)
+    nt = S.shape[0]
+    nxs = np.array([20, 20, -3])
+    nys = np.array([5, 5, 10])
+    S3 = np.stack([S[:, 0], 2 * S[:, 0], S[:, 1]], 1)
+    nxr = np.array([10, 10, -5, 30, 61])
+    nyr = np.array([4, 4, 12, 45, 20])
+    rec, F, _ = fwi_amd.solver_acoustic_SS(1, nxs, nys, nxr, nyr, S3, None, d, dt, c, 10, nt // 10, nt,
+                                           free_surface=0)
+    O = rb.oracle_solver("f32", 1, nxs, nys, nxr, nyr, S3, None, d, dt, c, 10, nt // 10, nt, free_surface=0)
+    assert np.array_equal(rec, O.rec)
+    assert np.array_equal(F, O.fields)
+    adj = np.asfortranarray(np.random.default_rng(0).standard_normal((nt, len(nxr))).astype(np.float32))
+    _, _, k = fwi_amd.solver_acoustic_SS(2, nxs, nys, nxr, nyr, S3, adj, d, dt, c, 10, nt // 10, nt,
+                                         Fields=O.fields.copy(order="F"), free_surface=0)
+    O2 = rb.oracle_solver("f32", 2, nxs, nys, nxr, nyr, S3, adj, d, dt, c, 10, nt // 10, nt, fields=O.fields,
+                          free_surface=0)
+    assert np.array_equal(k, O2.kernel)
+
+
+@pytest.mark.parametrize("vx,rows", [(1, 16), (2, 64), (4, 64), (4, 7), (4, 1000)])
+def test_batched_windows_vs_oracle(vx, rows):
+    """Several shots with uneven windows in one context; each shot equals the
+    reference solver run on its window sub-model."""
+    rng = np.random.default_rng(7)
+    ny, nx, nt, ss = 36, 420, 160, 10
+    vp = (1500 + 1500 * rng.random((ny, nx))).astype(np.float32)
+    shots, wav = [], []
+    for s in range(5):
+        w = int(rng.integers(40, 300))
+        x0 = int(rng.integers(0, nx - w + 1))
+        nr = int(rng.integers(1, 12))
+        shots.append(dict(x0=x0, w=w, src_x=[int(rng.integers(1, w + 1))], src_y=[int(rng.integers(1, ny + 1))],
+                          rec_x=rng.integers(1, w + 1, nr).tolist(), rec_y=rng.integers(1, ny + 1, nr).tolist()))
+        wav.append(synth.ricker(float(rng.uniform(8, 14)), nt, 0.002)[:, None])
+    d, dt = np.float32(20.0), np.float32(0.002)
+    ctx = fwi_amd.Context(ny, nx, d, dt, nt, ss, dpml=20, free_surface=0, vx=vx, rows_per_task=rows)
+    ctx.set_shots(shots)
+    ctx.set_model(vp)
+    ctx.set_sources(wav)
+    ctx.forward(store_snapshots=True)
+    syn = ctx.synthetic()
+    obs = [(0.9 * a + 0.01).astype(np.float32) for a in syn]
+    ctx.set_observed(obs)
+    mis = ctx.misfit()
+    adj = ctx.adjoint_sources()
+    ctx.adjoint()
+    kers = ctx.kernels()
+    scale = 1.0 / len(shots)
+    water = rng.integers(0, ny // 2, nx).astype(np.int32)
+    ctx.reduce_gradient(scale, water)
+    grad = ctx.get_gradient()
+    mis_o = 0.0
+    kers_o = []
+    for s, w, sy, a in zip(shots, wav, syn, adj):
+        sub = np.asfortranarray(vp[:, s["x0"]:s["x0"] + s["w"]])
+        O = rb.oracle_solver("f32", 1, s["src_x"], s["src_y"], s["rec_x"], s["rec_y"], w, None, d, dt, sub, ss,
+                             nt // ss, nt, free_surface=0)
+        assert np.array_equal(sy, O.rec), f"shot {s}: rec diff {np.abs(sy - O.rec).max()}"
+        m32, adj_o = rb.oracle_misfit(O.rec, 0.9 * O.rec + np.float32(0.01), float(dt))
+        m64, _ = rb.oracle_misfit(O.rec.astype(np.float64), (0.9 * O.rec + np.float32(0.01)).astype(np.float64),
+                                  float(dt), "f64")
+        assert np.array_equal(a, adj_o)
+        mis_o += m32
+        O2 = rb.oracle_solver("f32", 2, s["src_x"], s["src_y"], s["rec_x"], s["rec_y"], w, adj_o, d, dt, sub, ss,
+                              nt // ss, nt, fields=O.fields, free_surface=0)
+        kers_o.append(O2.kernel)
+    for k, ko in zip(kers, kers_o):
+        assert np.array_equal(k, ko), f"kernel diff {np.abs(k - ko).max()}"
+    assert abs(mis - mis_o) <= 1e-5 * abs(mis_o)
+    g_o = np.zeros((ny, nx))
+    for s, ko in zip(shots, kers_o):
+        g_o[:, s["x0"]:s["x0"] + s["w"]] += ko.astype(np.float64)
+    g_o *= scale
+    for i in range(nx):
+        g_o[:water[i], i] = 0.0
+    assert np.allclose(grad, g_o, rtol=1e-12, atol=1e-300)

@@ -1,0 +1,185 @@
+"""The CPU oracle against the golden vectors written from the reference's own
+Fortran (tests/golden/make_golden.py) and, where oracle/_ref is built,
+against the reference routines directly on fresh inputs."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import refbind as rb
+from fwi_amd import synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def _oracle_gradient_path(prec, fs, g):
+    dt_np = np.float64 if prec == "f64" else np.float32
+    d, dt, nt, ss = dt_np(g["d"]), dt_np(g["dt"]), int(g["nt"]), int(g["ss"])
+    S = g["S"].astype(dt_np)
+    args = (g["nxs"], g["nys"], g["nxr"], g["nyr"])
+    obs = rb.oracle_solver(prec, 3, *args, S, None, d, dt, g["c_true"], ss, nt // ss, nt, free_surface=fs).rec
+    F = rb.oracle_solver(prec, 1, *args, S, None, d, dt, g["c_init"], ss, nt // ss, nt, free_surface=fs)
+    mis, adj = rb.oracle_misfit(F.rec, obs, float(dt), prec)
+    A = rb.oracle_solver(prec, 2, *args, S, adj, d, dt, g["c_init"], ss, nt // ss, nt, fields=F.fields,
+                         free_surface=fs)
+    return obs, F, mis, adj, A
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_tiny_fixture(fs, prec):
+    g = _load(f"tiny_fs{fs}_r{'4' if prec == 'f32' else '8'}.npz")
+    obs, F, mis, adj, A = _oracle_gradient_path(prec, fs, g)
+    assert np.array_equal(obs, g["obs"])
+    assert np.array_equal(F.rec, g["rec"])
+    if "fields" in g:
+        assert np.array_equal(F.fields, g["fields"])
+    assert np.array_equal(adj, g["adj"])
+    assert mis == float(g["misfit"])
+    assert np.array_equal(A.kernel, g["kernel"])
+    assert np.abs(A.kernel).max() > 0
+
+
+@pytest.mark.slow
+def test_config1_fixture():
+    """Config-1 single-shot gradient path (150x400, nt=2000, CPML all round)."""
+    g = _load("config1_fs0_r4.npz")
+    acq = synth.config1(free_surface=0)
+    sh = acq.shots[0]
+    f32 = np.float32
+    args = (sh["src_x"], sh["src_y"], sh["rec_x"], sh["rec_y"])
+    obs = rb.oracle_solver("f32", 3, *args, acq.wavelets[0], None, f32(acq.h), f32(acq.dt), acq.vp_true, 10, 200,
+                           acq.nt, free_surface=0).rec
+    assert np.array_equal(obs[::10, ::10], g["obs_dec"])
+    F = rb.oracle_solver("f32", 1, *args, acq.wavelets[0], None, f32(acq.h), f32(acq.dt), acq.vp_init, 10, 200,
+                         acq.nt, free_surface=0)
+    assert np.array_equal(F.rec[::10, ::10], g["rec_dec"])
+    mis, adj = rb.oracle_misfit(F.rec, obs, float(acq.dt))
+    assert mis == float(g["misfit"])
+    assert np.array_equal(adj[::10, ::10], g["adj_dec"])
+    A = rb.oracle_solver("f32", 2, *args, acq.wavelets[0], adj, f32(acq.h), f32(acq.dt), acq.vp_init, 10, 200,
+                         acq.nt, fields=F.fields, free_surface=0)
+    assert np.array_equal(A.kernel, g["kernel"])
+
+
+@pytest.mark.parametrize("dpml", [20, 50])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_pml_fixture(dpml, prec):
+    g = _load(f"pml_d{dpml}_r{'4' if prec == 'f32' else '8'}.npz")
+    dt_np = np.float64 if prec == "f64" else np.float32
+    ct = C.c_double if prec == "f64" else C.c_float
+    dn = dpml + 6
+    fA = np.zeros(dn, dt_np)
+    fB = np.zeros(dn, dt_np)
+    mats = [np.zeros((dn, dn), dt_np, order="F") for _ in range(8)]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    getattr(rb.oracle(), f"oracle_pml_coeffs_{prec}")(3, dpml, ct(0.002), ct(25.0), dn, p(fA), p(fB),
+                                                      *[p(m) for m in mats])
+    assert np.array_equal(fA, g["fA"]) and np.array_equal(fB, g["fB"])
+    for name, m in zip(["ldA", "rdA", "luA", "ruA", "ldB", "rdB", "luB", "ruB"], mats):
+        assert np.array_equal(m, g[name]), name
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_fd_fixture(prec):
+    g = _load(f"fd_r{'4' if prec == 'f32' else '8'}.npz")
+    dt_np = np.float64 if prec == "f64" else np.float32
+    for num in (1, 2):
+        for acc in (1, 3, 4, 5):
+            if num == 2 and acc == 1:
+                continue          # not defined by the reference (coefficients left as passed in)
+            coef = np.zeros(acc + 1, dt_np)
+            getattr(rb.oracle(), f"oracle_fd_coef_{prec}")(num, acc, coef.ctypes.data_as(C.c_void_p))
+            assert np.array_equal(coef, g[f"n{num}_a{acc}"]), (num, acc)
+
+
+needs_ref = pytest.mark.skipif(not rb.ref_available("fs1_r4"), reason="reference build (oracle/_ref) not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_space_derivs_vs_reference(kind, prec):
+    rng = np.random.default_rng(kind)
+    dt_np = np.float64 if prec == "f64" else np.float32
+    ct_int = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    nyt, nxt = 23, 31
+    p = np.asfortranarray(rng.standard_normal((nyt, nxt)).astype(dt_np))
+    coef = np.zeros(4, dt_np)
+    getattr(rb.oracle(), f"oracle_fd_coef_{prec}")(1 if kind >= 2 else 2, 3, coef.ctypes.data_as(C.c_void_p))
+    out_o = np.zeros_like(p, order="F")
+    getattr(rb.oracle(), f"oracle_space_deriv_{prec}")(kind, 3, coef.ctypes.data_as(C.c_void_p), nyt, nxt,
+                                                       p.ctypes.data_as(C.c_void_p), out_o.ctypes.data_as(C.c_void_p))
+    lib = rb.ref("fs1_r8" if prec == "f64" else "fs1_r4")
+    fn = [lib.space_deriv2x_, lib.space_deriv2y_, lib.space_derivx_, lib.space_derivy_][kind]
+    out_r = np.zeros_like(p, order="F")
+    fn(ct_int(3), coef.ctypes.data_as(C.c_void_p), ct_int(nyt), ct_int(nxt), p.ctypes.data_as(C.c_void_p),
+       out_r.ctypes.data_as(C.c_void_p))
+    assert np.array_equal(out_o, out_r)
+
+
+@needs_ref
+@pytest.mark.parametrize("fs", [0, 1])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("seed", [11, 12])
+def test_solver_vs_reference_random(fs, prec, seed):
+    """Fresh random model, acquisition (including cells in the CPML frame and duplicate
+    cells) and adjoint sources; forward and adjoint must equal the reference exactly."""
+    rng = np.random.default_rng(seed)
+    ny, nx, nt, ss = int(rng.integers(20, 45)), int(rng.integers(30, 70)), 150, 10
+    dt_np = np.float64 if prec == "f64" else np.float32
+    c = (1500 + 2500 * rng.random((ny, nx))).astype(dt_np)
+    ns, nr = 3, 6
+    nxs, nys = rng.integers(-5, nx + 6, ns), rng.integers(1, ny + 6, ns)
+    nxr, nyr = rng.integers(-5, nx + 6, nr), rng.integers(1, ny + 6, nr)
+    nxr[1], nyr[1] = nxr[0], nyr[0]
+    S = rng.standard_normal((nt, ns)).astype(dt_np)
+    d, dt = dt_np(20.0), dt_np(0.0015)
+    v = f"fs{fs}_r{'8' if prec == 'f64' else '4'}"
+    R = rb.ref_solver(v, 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nt // ss, nt)
+    O = rb.oracle_solver(prec, 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nt // ss, nt, free_surface=fs)
+    assert np.array_equal(R.rec, O.rec) and np.array_equal(R.fields, O.fields)
+    adj = rng.standard_normal((nt, nr)).astype(dt_np)
+    k0 = rng.standard_normal((ny, nx)).astype(dt_np)
+    R2 = rb.ref_solver(v, 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nt // ss, nt, kernel=k0, fields=R.fields)
+    O2 = rb.oracle_solver(prec, 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nt // ss, nt, kernel=k0,
+                          fields=R.fields, free_surface=fs)
+    assert np.array_equal(R2.kernel, O2.kernel)
+    assert np.abs(R2.rec).max() == 0 and np.abs(O2.rec).max() == 0
+
+
+@needs_ref
+def test_forward_simple_vs_reference():
+    c, S, nxs, nys, nxr, nyr = synth.small_case(2, 30, 50, 150)
+    R = rb.ref_forward_simple(int(nxs[0]), int(nys[0]), nxr, nyr, S[:, 0], np.float32(25), np.float32(0.002), c, 150)
+    O = rb.oracle_forward_simple(int(nxs[0]), int(nys[0]), nxr, nyr, S[:, 0], np.float32(25), np.float32(0.002), c,
+                                 150)
+    assert np.abs(R).max() > 0
+    assert np.array_equal(R, O)
+
+
+@needs_ref
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_misfit_l2_core_vs_reference(prec):
+    rng = np.random.default_rng(5)
+    dt_np = np.float64 if prec == "f64" else np.float32
+    s = rng.standard_normal((97, 5)).astype(dt_np)
+    o = rng.standard_normal((97, 5)).astype(dt_np)
+    s[:, 2] = 0            # all-zero synthetic: not normalised, contributes zero
+    o[:, 3] = 0            # all-zero observed: skipped
+    m_o, a_o = rb.oracle_misfit(s, o, 0.004, prec)
+    # the reference Misfit_L2 on traces normalised as Misfit_function does
+    DS = np.zeros_like(s)
+    DT = np.zeros_like(o)
+    for j in range(5):
+        if np.any(o[:, j] != 0) and np.abs(s[:, j]).max() != 0 and np.abs(o[:, j]).max() != 0:
+            DS[:, j] = s[:, j] / np.abs(s[:, j]).max()
+            DT[:, j] = o[:, j] / np.abs(o[:, j]).max()
+    m_r, a_r = rb.ref_misfit_l2(DS, DT, 0.004, "fs1_r8" if prec == "f64" else "fs1_r4")
+    assert m_o == m_r
+    assert np.array_equal(a_o, a_r)
