@@ -38,7 +38,6 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=5000)
-    ap.add_argument("--vx", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--snap", choices=["f32", "bf16", "f16"], default="f32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -159,7 +158,7 @@ def main():
     snap_kind = {"f32": fwi_amd.SNAP_F32, "bf16": fwi_amd.SNAP_BF16, "f16": fwi_amd.SNAP_F16}[a.snap]
     snap_bytes = 4 if a.snap == "f32" else 2
     ctx = fwi_amd.Context(acq.ny, acq.nx, np.float32(acq.h), np.float32(acq.dt), acq.nt, acq.store_snap,
-                          dpml=acq.dpml, free_surface=acq.free_surface, snap_kind=snap_kind, vx=a.vx,
+                          dpml=acq.dpml, free_surface=acq.free_surface, snap_kind=snap_kind,
                           rows_per_task=a.rows)
     t_setup = time.perf_counter()
     ctx.set_shots(acq.shots)
@@ -217,8 +216,7 @@ def main():
             "nshots": len(acq.shots),
             "nt": acq.nt,
             "snapshots": a.snap,
-            "vx": a.vx or 4,
-            "rows_per_task": a.rows or 64,
+            "rows_per_task": a.rows or 32,
         },
         "seconds_per_gradient": wall / a.steps,
         "misfit": misfit,

@@ -79,10 +79,21 @@ def build_ref(verbose: bool = False) -> None:
         raise RuntimeError("reference build failed")
 
 
+def build_fortran(verbose: bool = False) -> None:
+    """Fortran drop-in test hosts and the ISO_C_BINDING module (tests/fortran), where flang exists."""
+    if not os.path.exists(os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")):
+        return
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "fortran")], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran host build failed")
+
+
 def build_all(verbose: bool = False) -> None:
     build_hip(verbose)
     build_oracle(verbose)
     build_ref(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":

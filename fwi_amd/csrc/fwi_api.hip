@@ -91,14 +91,10 @@ struct Arena {
 struct Shot {
     int x0 = 0, W = 0;
     std::vector<int> sx, sy, rx, ry;     // 1-based window coordinates
-    int NX = 0, P = 0, Ws = 0, ntile = 0;
-    std::vector<int> txs, txe;           // tile column ranges
-    // CSR tables (host), built per tiling
-    std::vector<int> iptr[2], rptr;
-    std::vector<int2> ient[2], rent;
+    int NX = 0, P = 0, Ws = 0, moff = 0;
     // arena offsets
     size_t o_p[2], o_phx[2], o_chx, o_phy[2], o_chy;                    // state arena
-    size_t o_c2, o_iptr[2], o_ient[2], o_rptr, o_rent;                  // static arena
+    size_t o_c2;                                                        // static arena
     size_t o_src, o_rec, o_obs, o_adj;                                  // trace arena
     size_t o_snap, o_grad;                                              // snapshot arena / grad arena
     size_t toff = 0;                                                    // receiver-trace element offset
@@ -106,18 +102,29 @@ struct Shot {
     int roff = 0;                                                       // first global receiver index
 };
 
+// A cell with a trace index, in padded coordinates.
+struct Cell {
+    int Y, X, tr;
+};
+
 }  // namespace
 
 struct fwi_ctx {
     fwi_config cfg{};
-    int dn = 0, NY = 0, nt_store = 0, vx = 4, rows = 64;
+    int dn = 0, NY = 0, nt_store = 0, rows = 32;
     float fA[fwi::MAXDN + 1], fB[fwi::MAXDN + 1];
     std::vector<Shot> shots;
     int nrec_total = 0, nsrc_total = 0;
     Arena state, stat, traces, snaps, grads, misc;
     fwi::ShotDev* d_desc = nullptr;
-    fwi::Task* d_tasks = nullptr;
-    int ntasks = 0;
+    fwi::Task* d_ftasks = nullptr;        // frame tasks
+    fwi::Task* d_itasks = nullptr;        // interior tasks
+    int nftasks = 0, nitasks = 0;
+    int* d_rows = nullptr;
+    int2* d_ents = nullptr;
+    float* d_c2s = nullptr;               // four shifted copies of the model c2 (interior loads)
+    int Pm = 0;
+    size_t c2s_stride = 0;
     float* d_vp = nullptr;
     fwi::C2Shot* d_c2shots = nullptr;
     fwi::RecMap* d_recmap = nullptr;
@@ -158,45 +165,51 @@ int validate_config(const fwi_config& c) {
     if (!(c.d > 0.0f) || !(c.dt > 0.0f)) return fail("d and dt must be positive");
     if (c.snap_kind != fwi::SNAP_F32 && c.snap_kind != fwi::SNAP_BF16 && c.snap_kind != fwi::SNAP_F16)
         return fail("unknown snapshot kind");
-    if (c.vx != 0 && c.vx != 1 && c.vx != 2 && c.vx != 4) return fail("vx must be 1, 2 or 4");
+    if (c.rows_per_task < 0) return fail("rows_per_task must be >= 0");
     return 0;
 }
 
-// Split the padded width into tiles of at most step_tile_cols(vx) columns.
-// Every tile starts at a multiple of 4 and at or before column NX-4, which
-// keeps the clamped stencil of the last columns inside the tile's LDS row.
-void make_tiles(Shot& s, int vx) {
-    const int tw = fwi::step_tile_cols(vx);
-    s.ntile = (s.NX + tw - 1) / tw;
-    s.txs.assign(s.ntile, 0);
-    s.txe.assign(s.ntile, 0);
-    for (int t = 0; t < s.ntile; ++t) s.txs[t] = t * tw;
-    if (s.ntile > 1 && s.txs[s.ntile - 1] > s.NX - 4) s.txs[s.ntile - 1] = (s.NX - 4) & ~3;
-    for (int t = 0; t < s.ntile; ++t) s.txe[t] = (t + 1 < s.ntile) ? s.txs[t + 1] : s.NX;
+// Column ranges of at most maxw columns covering [a, b).  When b is the
+// padded width NX, the last range starts at or before NX-4 so the clamped
+// stencil of the last columns stays inside the frame kernel's LDS row.
+std::vector<std::pair<int, int>> tile_range(int a, int b, int maxw, int NX) {
+    std::vector<std::pair<int, int>> out;
+    if (b <= a) return out;
+    const int n = (b - a + maxw - 1) / maxw;
+    std::vector<int> st(n);
+    for (int t = 0; t < n; ++t) st[t] = a + t * maxw;
+    if (b == NX && n > 1 && st[n - 1] > NX - 4) st[n - 1] = NX - 4;
+    for (int t = 0; t < n; ++t) out.push_back({st[t], t + 1 < n ? st[t + 1] : b});
+    return out;
 }
 
-int tile_of(const Shot& s, int X) {
-    for (int t = 0; t < s.ntile; ++t)
-        if (X >= s.txs[t] && X < s.txe[t]) return t;
-    return -1;
+// Row chunks of at most maxr rows covering [a, b), near-equal sizes.
+std::vector<std::pair<int, int>> row_range(int a, int b, int maxr) {
+    std::vector<std::pair<int, int>> out;
+    if (b <= a) return out;
+    const int n = (b - a + maxr - 1) / maxr;
+    for (int c = 0; c < n; ++c)
+        out.push_back({a + (int)((long long)(b - a) * c / n), a + (int)((long long)(b - a) * (c + 1) / n)});
+    return out;
 }
 
-// CSR over (row, tile) of cells with a trace index each.
-void build_csr(const Shot& s, int NY, const std::vector<std::pair<int, int>>& cells /* (Y*NX+X) */,
-               const std::vector<int>& trace, std::vector<int>& ptr, std::vector<int2>& ent) {
-    const int nc = NY * s.ntile;
-    ptr.assign(nc + 1, 0);
-    std::vector<int> cnt(nc, 0);
-    std::vector<int> key(cells.size());
-    for (size_t e = 0; e < cells.size(); ++e) {
-        const int Y = cells[e].first, X = cells[e].second;
-        key[e] = Y * s.ntile + tile_of(s, X);
-        cnt[key[e]]++;
+// Per-task row tables of one cell set: rows[off + r] .. rows[off + r + 1]
+// index the task's entries on row ya + r; bit r of the mask is set when the
+// row has entries.
+void attach_cells(fwi::Task& t, const std::vector<Cell>& cells, std::vector<int>& rows, std::vector<int2>& ents,
+                  int& off, unsigned long long& mask) {
+    const int nr = t.yb - t.ya;
+    std::vector<std::vector<int2>> per(nr);
+    for (const Cell& c : cells)
+        if (c.Y >= t.ya && c.Y < t.yb && c.X >= t.xs && c.X < t.xe) per[c.Y - t.ya].push_back(int2{c.X, c.tr});
+    off = (int)rows.size();
+    mask = 0ull;
+    for (int r = 0; r < nr; ++r) {
+        rows.push_back((int)ents.size());
+        if (!per[r].empty()) mask |= 1ull << r;
+        for (const int2& e : per[r]) ents.push_back(e);
     }
-    for (int c = 0; c < nc; ++c) ptr[c + 1] = ptr[c] + cnt[c];
-    ent.assign(cells.size(), int2{0, 0});
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (size_t e = 0; e < cells.size(); ++e) ent[fill[key[e]]++] = int2{cells[e].second, trace[e]};
+    rows.push_back((int)ents.size());
 }
 
 int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
@@ -205,6 +218,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     const int dn = c->dn, NY = c->NY, ny = cf.ny;
     c->shots.assign(nshots, Shot());
     c->nrec_total = c->nsrc_total = 0;
+    std::vector<fwi::Task> ftasks, itasks;
+    std::vector<int> rows;
+    std::vector<int2> ents;
+    const int R = std::min(c->rows, fwi::MAX_TASK_ROWS);
     for (int i = 0; i < nshots; ++i) {
         Shot& s = c->shots[i];
         const fwi_shot& f = in[i];
@@ -218,34 +235,54 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         s.rx.assign(f.rec_x, f.rec_x + f.nrec);
         s.ry.assign(f.rec_y, f.rec_y + f.nrec);
         s.NX = s.W + 2 * dn;
-        s.P = (int)align_up(s.NX, 64);
-        s.Ws = (int)align_up(s.W, 4);
-        make_tiles(s, c->vx);
+        s.P = (int)align_up(s.NX + 8, 64);
+        s.moff = dn & 3;
+        s.Ws = (int)align_up(s.W + s.moff + 3, 4);
         auto in_grid = [&](int x, int y) { return x >= 1 - dn && x <= s.W + dn && y >= 1 - dn && y <= ny + dn; };
         for (size_t j = 0; j < s.sx.size(); ++j)
             if (!in_grid(s.sx[j], s.sy[j])) return fail("shot " + std::to_string(i) + ": source outside the padded grid");
         for (size_t j = 0; j < s.rx.size(); ++j)
             if (!in_grid(s.rx[j], s.ry[j])) return fail("shot " + std::to_string(i) + ": receiver outside the padded grid");
-        // injection tables: cells inside the model only (the reference writes
-        // its source array everywhere but only the interior update reads it);
-        // a cell listed twice keeps its last trace (solver_acoustic_SS.f90:320-329)
+
+        // injection cells: inside the model only (the reference writes its
+        // source array everywhere but only the interior update reads it); a
+        // cell listed twice keeps its last trace (solver_acoustic_SS.f90:320-329)
+        std::vector<Cell> inj[2], rec;
         for (int dir = 0; dir < 2; ++dir) {
             const std::vector<int>& X = dir == 0 ? s.sx : s.rx;
             const std::vector<int>& Y = dir == 0 ? s.sy : s.ry;
             std::map<std::pair<int, int>, int> last;
             for (size_t j = 0; j < X.size(); ++j)
                 if (X[j] >= 1 && X[j] <= s.W && Y[j] >= 1 && Y[j] <= ny) last[{dn + Y[j] - 1, dn + X[j] - 1}] = (int)j;
-            std::vector<std::pair<int, int>> cells;
-            std::vector<int> tr;
-            for (auto& kv : last) { cells.push_back(kv.first); tr.push_back(kv.second); }
-            build_csr(s, NY, cells, tr, s.iptr[dir], s.ient[dir]);
+            for (auto& kv : last) inj[dir].push_back(Cell{kv.first.first, kv.first.second, kv.second});
         }
-        {
-            std::vector<std::pair<int, int>> cells;
-            std::vector<int> tr;
-            for (size_t j = 0; j < s.rx.size(); ++j) { cells.push_back({dn + s.ry[j] - 1, dn + s.rx[j] - 1}); tr.push_back((int)j); }
-            build_csr(s, NY, cells, tr, s.rptr, s.rent);
+        for (size_t j = 0; j < s.rx.size(); ++j) rec.push_back(Cell{dn + s.ry[j] - 1, dn + s.rx[j] - 1, (int)j});
+
+        auto add_task = [&](std::vector<fwi::Task>& list, int xs, int xe, int ya, int yb) {
+            fwi::Task t;
+            std::memset(&t, 0, sizeof(t));
+            t.shot = i; t.xs = xs; t.xe = xe; t.ya = ya; t.yb = yb;
+            for (int dir = 0; dir < 2; ++dir) attach_cells(t, inj[dir], rows, ents, t.inj_off[dir], t.inj_mask[dir]);
+            attach_cells(t, rec, rows, ents, t.rec_off, t.rec_mask);
+            list.push_back(t);
+        };
+        // interior: model rows, 4-aligned columns [XL, XR) away from the frame
+        const int XR = s.NX - dn;
+        const int XL = std::min((int)align_up(dn, 4), XR);
+        for (auto& yr : row_range(dn, NY - dn, R))
+            for (auto& xr : tile_range(XL, XR, fwi::interior_tile_cols(), -1)) add_task(itasks, xr.first, xr.second, yr.first, yr.second);
+        // frame: left and right bands over the model rows, bottom (and top) bands over all columns
+        const int FW = fwi::frame_tile_cols();
+        for (auto& yr : row_range(dn, NY - dn, R)) {
+            for (auto& xr : tile_range(0, XL, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+            for (auto& xr : tile_range(XR, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
         }
+        for (auto& yr : row_range(NY - dn, NY, R))
+            for (auto& xr : tile_range(0, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+        if (!cf.free_surface)
+            for (auto& yr : row_range(0, dn, R))
+                for (auto& xr : tile_range(0, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+
         s.roff = c->nrec_total;
         s.toff = (size_t)c->nrec_total * cf.nt;
         s.soff = (size_t)c->nsrc_total * cf.nt;
@@ -263,15 +300,14 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         for (int b = 0; b < 2; ++b) s.o_phy[b] = c->state.add(yb);
         s.o_chy = c->state.add(yb);
         s.o_c2 = c->stat.add(field_bytes);
-        for (int d = 0; d < 2; ++d) {
-            s.o_iptr[d] = c->stat.add(s.iptr[d].size() * sizeof(int));
-            s.o_ient[d] = c->stat.add(std::max<size_t>(1, s.ient[d].size()) * sizeof(int2));
-        }
-        s.o_rptr = c->stat.add(s.rptr.size() * sizeof(int));
-        s.o_rent = c->stat.add(std::max<size_t>(1, s.rent.size()) * sizeof(int2));
         s.o_snap = c->snaps.add((size_t)(c->nt_store + 1) * ny * s.Ws * c->snap_elem());
         s.o_grad = c->grads.add((size_t)ny * s.Ws * sizeof(float));
     }
+    c->Pm = (int)align_up(cf.nx + 8, 64);
+    c->c2s_stride = align_up(8 + (size_t)ny * c->Pm, 64);
+    const size_t o_c2s = c->stat.add(4 * c->c2s_stride * sizeof(float));
+    const size_t o_rows = c->stat.add(std::max<size_t>(1, rows.size()) * sizeof(int));
+    const size_t o_ents = c->stat.add(std::max<size_t>(1, ents.size()) * sizeof(int2));
     const size_t ntr_rec = std::max<size_t>(1, (size_t)c->nrec_total * cf.nt);
     const size_t ntr_src = std::max<size_t>(1, (size_t)c->nsrc_total * cf.nt);
     const size_t o_src = c->traces.add(ntr_src * sizeof(float));
@@ -284,28 +320,17 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         s.o_obs = o_obs + s.toff * sizeof(float);
         s.o_adj = o_adj + s.toff * sizeof(float);
     }
-
-    // tasks: per shot, row chunks of about `rows` rows, then tiles
-    const int y0 = cf.free_surface ? dn : 0;
-    std::vector<fwi::Task> tasks;
-    for (int si = 0; si < nshots; ++si) {
-        const Shot& s = c->shots[si];
-        const int nrows = NY - y0;
-        const int nch = std::max(1, (nrows + c->rows - 1) / c->rows);
-        for (int ch = 0; ch < nch; ++ch) {
-            const int ya = y0 + (int)((long long)nrows * ch / nch), yb = y0 + (int)((long long)nrows * (ch + 1) / nch);
-            for (int t = 0; t < s.ntile; ++t) tasks.push_back(fwi::Task{si, s.txs[t], s.txe[t], ya, yb, t, 0, 0});
-        }
-    }
-    c->ntasks = (int)tasks.size();
+    c->nftasks = (int)ftasks.size();
+    c->nitasks = (int)itasks.size();
 
     // misc arena: descriptors, tasks, misfit scratch, gradient
     const int nblk = 64;
     c->nblk_misfit = nblk;
     const size_t o_desc = c->misc.add(nshots * sizeof(fwi::ShotDev));
-    const size_t o_tasks = c->misc.add(std::max<size_t>(1, tasks.size()) * sizeof(fwi::Task));
+    const size_t o_ft = c->misc.add(std::max<size_t>(1, ftasks.size()) * sizeof(fwi::Task));
+    const size_t o_it = c->misc.add(std::max<size_t>(1, itasks.size()) * sizeof(fwi::Task));
     const size_t o_vp = c->misc.add((size_t)cf.ny * cf.nx * sizeof(float));
-    const size_t o_c2s = c->misc.add(nshots * sizeof(fwi::C2Shot));
+    const size_t o_c2sh = c->misc.add(nshots * sizeof(fwi::C2Shot));
     const size_t o_rmap = c->misc.add(std::max(1, c->nrec_total) * sizeof(fwi::RecMap));
     const size_t o_rs = c->misc.add(nshots * sizeof(fwi::ResShot));
     const size_t o_gs = c->misc.add(nshots * sizeof(fwi::GradShot));
@@ -324,9 +349,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     HIPCHK(hipMemset(c->stat.base, 0, c->stat.size));
 
     c->d_desc = c->misc.at<fwi::ShotDev>(o_desc);
-    c->d_tasks = c->misc.at<fwi::Task>(o_tasks);
+    c->d_ftasks = c->misc.at<fwi::Task>(o_ft);
+    c->d_itasks = c->misc.at<fwi::Task>(o_it);
     c->d_vp = c->misc.at<float>(o_vp);
-    c->d_c2shots = c->misc.at<fwi::C2Shot>(o_c2s);
+    c->d_c2shots = c->misc.at<fwi::C2Shot>(o_c2sh);
     c->d_recmap = c->misc.at<fwi::RecMap>(o_rmap);
     c->d_resshots = c->misc.at<fwi::ResShot>(o_rs);
     c->d_gradshots = c->misc.at<fwi::GradShot>(o_gs);
@@ -335,6 +361,9 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     c->d_shot_misfit = c->misc.at<double>(o_sm);
     c->d_grad = c->misc.at<double>(o_grad);
     c->d_water = c->misc.at<int>(o_water);
+    c->d_c2s = c->stat.at<float>(o_c2s);
+    c->d_rows = c->stat.at<int>(o_rows);
+    c->d_ents = c->stat.at<int2>(o_ents);
 
     // descriptors
     std::vector<fwi::ShotDev> desc(nshots);
@@ -346,7 +375,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         Shot& s = c->shots[si];
         fwi::ShotDev& d = desc[si];
         std::memset(&d, 0, sizeof(d));
-        d.NX = s.NX; d.P = s.P; d.W = s.W; d.Ws = s.Ws; d.ntile = s.ntile;
+        d.NX = s.NX; d.P = s.P; d.W = s.W; d.Ws = s.Ws; d.moff = s.moff;
         d.ntr_src = (int)s.sx.size(); d.nrec = (int)s.rx.size();
         for (int b = 0; b < 2; ++b) {
             d.p[b] = c->state.at<float>(s.o_p[b]) + 4;
@@ -356,16 +385,16 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         d.chx = c->state.at<float>(s.o_chx);
         d.chy = c->state.at<float>(s.o_chy);
         d.c2 = c->stat.at<float>(s.o_c2) + 4;
-        for (int k = 0; k < 2; ++k) {
-            d.inj_ptr[k] = c->stat.at<int>(s.o_iptr[k]);
-            d.inj_ent[k] = c->stat.at<int2>(s.o_ient[k]);
-        }
+        // interior c2: padded column x of this window is model column x + o,
+        // read from the copy shifted by q = (-o) mod 4 so that 4-aligned x
+        // gives 16-byte aligned loads
+        const int o = s.x0 - dn;
+        const int q = ((-o) % 4 + 4) % 4;
+        d.c2i = c->d_c2s + (size_t)q * c->c2s_stride + 4 + o + q;
         d.inj_tr[0] = c->traces.at<float>(s.o_src);
         d.inj_tr[1] = c->traces.at<float>(s.o_adj);
         d.ntr[0] = (int)s.sx.size();
         d.ntr[1] = (int)s.rx.size();
-        d.rec_ptr = c->stat.at<int>(s.o_rptr);
-        d.rec_ent = c->stat.at<int2>(s.o_rent);
         d.rec_tr = c->traces.at<float>(s.o_rec);
         d.snap = c->snaps.at<void>(s.o_snap);
         d.snap_scale = nullptr;
@@ -374,18 +403,13 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         c2s[si] = fwi::C2Shot{const_cast<float*>(d.c2), s.NX, s.P, s.W, s.x0};
         for (int r = 0; r < (int)s.rx.size(); ++r) rmap[s.roff + r] = fwi::RecMap{s.toff, r, (int)s.rx.size()};
         rs[si] = fwi::ResShot{s.toff, s.roff, (int)s.rx.size()};
-        gs[si] = fwi::GradShot{d.grad, s.x0, s.W, s.Ws, 0};
-        for (int k = 0; k < 2; ++k) {
-            HIPCHK(hipMemcpy(c->stat.base + s.o_iptr[k], s.iptr[k].data(), s.iptr[k].size() * sizeof(int), hipMemcpyHostToDevice));
-            if (!s.ient[k].empty())
-                HIPCHK(hipMemcpy(c->stat.base + s.o_ient[k], s.ient[k].data(), s.ient[k].size() * sizeof(int2), hipMemcpyHostToDevice));
-        }
-        HIPCHK(hipMemcpy(c->stat.base + s.o_rptr, s.rptr.data(), s.rptr.size() * sizeof(int), hipMemcpyHostToDevice));
-        if (!s.rent.empty())
-            HIPCHK(hipMemcpy(c->stat.base + s.o_rent, s.rent.data(), s.rent.size() * sizeof(int2), hipMemcpyHostToDevice));
+        gs[si] = fwi::GradShot{d.grad, s.x0, s.W, s.Ws, s.moff};
     }
     HIPCHK(hipMemcpy(c->d_desc, desc.data(), nshots * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
-    if (!tasks.empty()) HIPCHK(hipMemcpy(c->d_tasks, tasks.data(), tasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
+    if (!ftasks.empty()) HIPCHK(hipMemcpy(c->d_ftasks, ftasks.data(), ftasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
+    if (!itasks.empty()) HIPCHK(hipMemcpy(c->d_itasks, itasks.data(), itasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
+    if (!rows.empty()) HIPCHK(hipMemcpy(c->d_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ents.empty()) HIPCHK(hipMemcpy(c->d_ents, ents.data(), ents.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_c2shots, c2s.data(), nshots * sizeof(fwi::C2Shot), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_recmap, rmap.data(), rmap.size() * sizeof(fwi::RecMap), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_resshots, rs.data(), nshots * sizeof(fwi::ResShot), hipMemcpyHostToDevice));
@@ -406,8 +430,9 @@ int run(fwi_ctx* c, int mode, bool store) {
     fwi::StepParams prm;
     std::memset(&prm, 0, sizeof(prm));
     prm.shots = c->d_desc;
-    prm.tasks = c->d_tasks;
-    prm.ntasks = c->ntasks;
+    prm.rows = c->d_rows;
+    prm.ents = c->d_ents;
+    prm.Pm = c->Pm;
     prm.NY = c->NY;
     prm.dn = c->dn;
     prm.ny = cf.ny;
@@ -446,7 +471,12 @@ int run(fwi_ctx* c, int mode, bool store) {
                 prm.slot = slot;
             }
         }
-        HIPCHK(fwi::launch_step(c->vx, prm, c->stream));
+        prm.tasks = c->d_ftasks;
+        prm.ntasks = c->nftasks;
+        HIPCHK(fwi::launch_frame(prm, c->stream));
+        prm.tasks = c->d_itasks;
+        prm.ntasks = c->nitasks;
+        HIPCHK(fwi::launch_interior(prm, c->stream));
     }
     if (c->timing) {
         HIPCHK(hipEventRecord(ev1, c->stream));
@@ -513,8 +543,7 @@ int ctx_init(fwi_ctx* c, const fwi_config* cfg, int nt_store) {
     c->dn = cfg->dpml + 2 * fwi::ACC;
     c->NY = cfg->ny + 2 * c->dn;
     c->nt_store = nt_store;
-    c->vx = cfg->vx ? cfg->vx : 4;
-    c->rows = cfg->rows_per_task > 0 ? cfg->rows_per_task : 64;
+    c->rows = cfg->rows_per_task > 0 ? std::min(cfg->rows_per_task, fwi::MAX_TASK_ROWS) : 32;
     cpml_table(cfg->dpml, cfg->dt, cfg->d, c->dn, c->fA, c->fB);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     return 0;
@@ -550,6 +579,7 @@ int fwi_set_model(fwi_ctx* c, const float* vp) {
     HIPCHK(hipMemcpyAsync(c->d_vp, vp, (size_t)cf.ny * cf.nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, cf.nx, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
                                 c->stream));
+    HIPCHK(fwi::launch_build_c2_shared(c->d_vp, cf.ny, cf.nx, cf.dt, cf.d, c->d_c2s, c->Pm, c->c2s_stride, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_model = true;
     return 0;
@@ -647,7 +677,7 @@ int fwi_get_kernels(fwi_ctx* c, float* out) {
         std::vector<float> k((size_t)ny * s.Ws);
         HIPCHK(hipMemcpy(k.data(), c->grads.base + s.o_grad, k.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (int i = 0; i < s.W; ++i)
-            for (int j = 0; j < ny; ++j) out[off + (size_t)i * ny + j] = k[(size_t)j * s.Ws + i];
+            for (int j = 0; j < ny; ++j) out[off + (size_t)i * ny + j] = k[(size_t)j * s.Ws + i + s.moff];
         off += (size_t)ny * s.W;
     }
     return 0;
@@ -742,7 +772,7 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
                                  slot_elems * sizeof(float), hipMemcpyDeviceToHost));
                 float* F = Fields + (size_t)m * nx * ny;
                 for (int i = 0; i < nx; ++i)
-                    for (int j = 0; j < ny; ++j) F[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i];
+                    for (int j = 0; j < ny; ++j) F[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i + s.moff];
             }
         }
     } else {
@@ -751,20 +781,20 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
         for (int m = 0; m <= nt_Store; ++m) {
             const float* F = Fields + (size_t)m * nx * ny;
             for (int i = 0; i < nx; ++i)
-                for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i] = F[(size_t)i * ny + j];
+                for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i + s.moff] = F[(size_t)i * ny + j];
             HIPCHK(hipMemcpy(c.snaps.base + s.o_snap + (size_t)m * slot_elems * sizeof(float), slot.data(),
                              slot_elems * sizeof(float), hipMemcpyHostToDevice));
         }
         std::fill(slot.begin(), slot.end(), 0.0f);
         for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i] = kernel[(size_t)i * ny + j];
+            for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i + s.moff] = kernel[(size_t)i * ny + j];
         HIPCHK(hipMemcpy(c.grads.base + s.o_grad, slot.data(), slot_elems * sizeof(float), hipMemcpyHostToDevice));
         c.have_snaps = true;
         CHK(run(&c, 1, false));
         HIPCHK(hipStreamSynchronize(c.stream));
         HIPCHK(hipMemcpy(slot.data(), c.grads.base + s.o_grad, slot_elems * sizeof(float), hipMemcpyDeviceToHost));
         for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j) kernel[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i];
+            for (int j = 0; j < ny; ++j) kernel[(size_t)i * ny + j] = slot[(size_t)j * s.Ws + i + s.moff];
     }
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;

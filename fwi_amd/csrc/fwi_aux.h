@@ -22,13 +22,17 @@ struct ResShot {      // one shot's trace block
     int roff, nrec;
 };
 
-struct GradShot {     // one shot's FP32 imaging kernel
+struct GradShot {     // one shot's FP32 imaging kernel, model column m at k[row*Ws + m + moff]
     const float* k;
-    int x0, W, Ws, pad;
+    int x0, W, Ws, moff;
 };
 
 hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st);
+// Four copies of the model's c2 (copy q shifted right by q columns, row pitch
+// Pm, 4 floats of front padding) for the aligned interior loads of any window.
+hipError_t launch_build_c2_shared(const float* vp, int ny, int nxm, float dt, float d, float* out, int Pm,
+                                  size_t copy_stride, hipStream_t st);
 hipError_t launch_misfit(const float* synth, const float* obs, const RecMap* map, int nrec_total, const ResShot* shots,
                          int nshots, int nt, float dt, float* invs, float* adj, double* partial, int nblk_x,
                          double* shot_misfit, hipStream_t st);

@@ -25,6 +25,23 @@ __global__ void build_c2_kernel(const float* __restrict__ vp, int ny, int nxm, c
     }
 }
 
+// Shared interior c2: out + q*copy_stride + 4 holds c2(row, j - q) at [row*Pm + j].
+__global__ void build_c2_shared_kernel(const float* __restrict__ vp, int ny, int nxm, float dt, float d,
+                                       float* __restrict__ out, int Pm, size_t copy_stride) {
+    const int q = blockIdx.y;
+    float* base = out + q * copy_stride + 4;
+    const size_t n = (size_t)ny * Pm;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / Pm), j = (int)(e % Pm) - q;
+        float v = 0.0f;
+        if (j >= 0 && j < nxm) {
+            const float c = vp[(size_t)j * ny + r];
+            v = ((c * c) * (dt * dt)) / (d * d);
+        }
+        base[e] = v;
+    }
+}
+
 // Per receiver: max |synth|, max |obs| and whether obs has a non-zero sample
 // (Misfit.f90:31-51, is_not_cero in auxiliar.f90:282-299).  Traces are
 // time-major [t][nrec_s] per shot; g indexes receivers over all shots.
@@ -100,7 +117,7 @@ __global__ void gradient_reduce_kernel(const GradShot* __restrict__ shots, int n
         for (int s = 0; s < nshots; ++s) {
             const GradShot g = shots[s];
             const int xm = x - g.x0;
-            if (xm >= 0 && xm < g.W) acc += (double)g.k[(size_t)y * g.Ws + xm];
+            if (xm >= 0 && xm < g.W) acc += (double)g.k[(size_t)y * g.Ws + xm + g.moff];
         }
         double v = acc * scale;
         if (water_rows && y < water_rows[x]) v = 0.0;
@@ -114,6 +131,13 @@ hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots
                            float dt, float d, hipStream_t st) {
     if (nshots == 0) return hipSuccess;
     hipLaunchKernelGGL(build_c2_kernel, dim3(256, nshots), dim3(256), 0, st, vp, ny, nxm, shots, dn, NY, dt, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_c2_shared(const float* vp, int ny, int nxm, float dt, float d, float* out, int Pm,
+                                  size_t copy_stride, hipStream_t st) {
+    hipLaunchKernelGGL(build_c2_shared_kernel, dim3(256, 4), dim3(256), 0, st, vp, ny, nxm, dt, d, out, Pm,
+                       copy_stride);
     return hipGetLastError();
 }
 

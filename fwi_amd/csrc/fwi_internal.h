@@ -8,33 +8,34 @@ namespace fwi {
 
 constexpr int ACC = 3;        // stencil radius (solver_acoustic_SS.f90:91, acc=3)
 constexpr int MAXDN = 96;     // largest CPML frame dn = dpml + 2*acc supported
+constexpr int MAX_TASK_ROWS = 64;   // rows per task (bitmask width)
 
 // Snapshot storage precisions.
 enum SnapKind : int { SNAP_F32 = 0, SNAP_BF16 = 1, SNAP_F16 = 2 };
 
-// Per-shot device descriptor.  Every wavefield-shaped array is row-major with
-// x (horizontal) contiguous: element (y, x) of a padded field lives at
-// base[y * P + x], 0 <= y < NY, 0 <= x < NX, and the allocation extends 4
-// floats before row 0 and past the last row so the vector loads of the edge
-// lanes stay inside it.  The model window (rows dn..dn+ny-1, columns
-// dn..dn+W-1) is stored separately for snapshots and the imaging kernel as
-// [ny][Ws] with model column m at offset m.
+// Per-shot device descriptor.
+//
+// Padded fields are row-major with x contiguous: element (y, x) lives at
+// base[y * P + x], 0 <= y < NY, 0 <= x < NX; each allocation extends 4
+// floats before row 0 so the vector loads of halo lanes stay inside it, and
+// the pitch P (a multiple of 64) leaves at least 8 spare columns per row.
+//
+// The model window (rows dn..dn+ny-1, columns dn..dn+W-1) is stored for the
+// snapshots and the imaging kernel as [ny][Ws]: model column m of a row lives
+// at offset m + moff with moff = dn % 4, so padded column x maps to storage
+// column x - dn + moff, which has the same residue mod 4 as x (aligned float4).
 struct ShotDev {
     int NX, P, W, Ws;             // padded width, field pitch, model width, model-window pitch
-    int ntile;                    // number of x tiles (per launch configuration)
-    int ntr_src, nrec, pad0;      // source traces, receivers
-    float* p[2];                  // two time levels; p[(k&1)] holds p at time k-1 when stepping k
-    const float* c2;              // c^2 dt^2 / d^2 on the padded grid
+    int moff, ntr_src, nrec, pad0;
+    float* p[2];                  // two time levels; p[k & 1] holds p at time k-1 when stepping k
+    const float* c2;              // c^2 dt^2 / d^2 on the padded grid (frame cells)
+    const float* c2i;             // c2 of the interior, row y-dn of the shared model copy, indexed by x
     float* phx[2];                // x-band CPML phi, ping-pong, [2 bands][NY][dn]
     float* chx;                   // x-band CPML chi, [2][NY][dn]
     float* phy[2];                // y-band CPML phi, ping-pong, [2 bands][dn][P]
     float* chy;                   // y-band CPML chi, [2][dn][P]
-    const int* inj_ptr[2];        // CSR over (row, tile) of injection cells: [0] forward, [1] adjoint
-    const int2* inj_ent[2];       // {padded column, trace index}
-    const float* inj_tr[2];       // injected traces, time-major [t][ntr]
+    const float* inj_tr[2];       // injected traces, time-major [t][ntr]: [0] sources, [1] adjoint sources
     int ntr[2];
-    const int* rec_ptr;           // CSR over (row, tile) of receiver cells
-    const int2* rec_ent;          // {padded column, receiver index}
     float* rec_tr;                // recorded traces, time-major [t][nrec]
     void* snap;                   // snapshot slots [slot][ny][Ws]
     const float* snap_scale;      // per-slot scale factors (compressed kinds), or null
@@ -42,35 +43,48 @@ struct ShotDev {
     size_t snap_stride;           // elements per slot
 };
 
-// A unit of work: one x tile of one shot over rows [ya, yb).
+// A unit of work: columns [xs, xe) of one shot over rows [ya, yb), at most
+// MAX_TASK_ROWS rows.  Injection and receiver cells of the task are listed
+// per row: rows[off + (y - ya)] .. rows[off + (y - ya) + 1] index the entry
+// array; the masks flag the rows that have entries.
 struct Task {
-    int shot, xs, xe, ya, yb, tile, pad0, pad1;
+    int shot, xs, xe, ya, yb;
+    int inj_off[2];               // [0] forward sources, [1] adjoint (receiver cells)
+    int rec_off;
+    unsigned long long inj_mask[2];
+    unsigned long long rec_mask;
 };
 
-// Per-launch parameters of the time-step kernel.
+// Per-launch parameters of the time-step kernels.
 struct StepParams {
     const ShotDev* shots;
     const Task* tasks;
+    const int* rows;              // per-task row pointers into ents
+    const int2* ents;             // {padded column, trace index}
     int ntasks;
     int NY, dn, ny;
-    int k;              // Fortran time index of the step being taken (2..nt)
-    int mode;           // 0 forward (inject sources, record receivers), 1 adjoint (inject at receivers)
-    int snap_op;        // 0 none, 1 store snapshot slot, 2 correlate with slot
+    int Pm;                       // pitch of the shared interior c2 rows
+    int k;                        // Fortran time index of the step being taken (2..nt)
+    int mode;                     // 0 forward (inject sources, record receivers), 1 adjoint (inject at receivers)
+    int snap_op;                  // 0 none, 1 store snapshot slot, 2 correlate with slot
     int slot;
-    int snap_kind;      // SnapKind
-    int record;         // 1: sample receivers after the update
+    int snap_kind;                // SnapKind
+    int record;                   // 1: sample receivers after the update
     float dt;
-    float ssf;          // store_snap as float
+    float ssf;                    // store_snap as float
     int free_surface;
-    int pad0;
-    float fA[MAXDN + 1];   // CPML factors indexed by max(dx, dy), 1-based distance into the frame
+    float fA[MAXDN + 1];          // CPML factors indexed by max(dx, dy), 1-based distance into the frame
     float fB[MAXDN + 1];
 };
 
-// Launch configurations of the step kernel (cells per lane).
-constexpr int step_halo_lanes(int vx) { return (ACC + vx - 1) / vx; }
-constexpr int step_tile_cols(int vx) { return (64 - 2 * step_halo_lanes(vx)) * vx; }
+// Geometry of the two kernels.
+constexpr int FRAME_VX = 1;                                         // frame kernel: 1 column per lane
+constexpr int frame_halo_lanes() { return (ACC + FRAME_VX - 1) / FRAME_VX; }
+constexpr int frame_tile_cols() { return (64 - 2 * frame_halo_lanes()) * FRAME_VX; }
+constexpr int INTERIOR_VX = 4;                                      // interior kernel: 4 columns per lane
+constexpr int interior_tile_cols() { return (64 - 2) * INTERIOR_VX; }   // lanes 0 and 63 only load halo
 
-hipError_t launch_step(int vx, const StepParams& prm, hipStream_t stream);
+hipError_t launch_frame(const StepParams& prm, hipStream_t stream);
+hipError_t launch_interior(const StepParams& prm, hipStream_t stream);
 
 }  // namespace fwi

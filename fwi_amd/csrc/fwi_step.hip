@@ -1,16 +1,19 @@
-// Fused time-step kernel for the 2-D acoustic CPML propagator (gfx950).
+// Time-step kernels of the 2-D acoustic CPML propagator (gfx950).
 //
-// One launch advances every shot of a batch by one time step.  The work unit
-// (Task) is one x tile of one shot over a range of rows.  A 64-lane wavefront
-// owns the tile: lane l holds VX consecutive columns, the first and last H
-// lanes only load halo columns.  The wavefront marches down its rows keeping
-// the 7 rows p(yc-3..yc+3) of the current time level in registers (y
-// stencil), stages the current row in LDS (x stencil), and in the same pass
-//   * updates the CPML memory variables of the cells in the absorbing frame,
-//   * injects source (or adjoint source) samples,
-//   * writes p at the new time level over p at the previous-previous level,
-//   * samples receivers,
-//   * stores the forward snapshot or applies the imaging condition.
+// Each time step is two launches over disjoint cells of every shot in the
+// batch:
+//   interior_kernel  cells of the model window away from the CPML frame:
+//                    4 columns per lane, float4 loads, x stencil through DPP
+//                    wave shifts, y stencil from an 8-slot register window
+//                    rotated by unrolling, shared c2 rows, snapshot/imaging
+//                    and injection/receivers fused in;
+//   frame_kernel     the CPML frame (and the two model columns between the
+//                    frame and the first 4-aligned interior column): the
+//                    general update with clamped derivatives, the CPML memory
+//                    variables and the corner terms.
+// Both write p at the new time level over p at time k-2 (each cell only
+// reads its own old value) and read p at time k-1, which no launch of the
+// step writes.
 //
 // Arithmetic follows the reference statement by statement so results match it
 // bit for bit (the file is compiled with -ffp-contract=off):
@@ -33,6 +36,8 @@
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+
+#include <type_traits>
 
 namespace fwi {
 namespace {
@@ -65,24 +70,19 @@ __device__ __forceinline__ float dd1(float m3, float m2, float m1, float p1, flo
     return s;
 }
 
-template <int VX> __device__ __forceinline__ void vload(float (&d)[VX], const float* src) {
-    if constexpr (VX == 4) {
-        const float4 t = *reinterpret_cast<const float4*>(src);
-        d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
-    } else if constexpr (VX == 2) {
-        const float2 t = *reinterpret_cast<const float2*>(src);
-        d[0] = t.x; d[1] = t.y;
-    } else {
-        d[0] = *src;
-    }
-}
-
-template <int VX> __device__ __forceinline__ void vzero(float (&d)[VX]) {
-#pragma unroll
-    for (int i = 0; i < VX; ++i) d[i] = 0.0f;
-}
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Value held by lane-1 (0 in lane 0) and by lane+1 (0 in lane 63): DPP
+// wave_shr:1 / wave_shl:1.  Called with all 64 lanes active.
+__device__ __forceinline__ float from_left(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
 
 __device__ __forceinline__ void snap_store(void* base, size_t idx, float v, int kind, float inv_scale) {
     if (kind == SNAP_F32) {
@@ -100,28 +100,37 @@ __device__ __forceinline__ float snap_load(const void* base, size_t idx, int kin
     return __half2float(static_cast<const __half*>(base)[idx]) * scale;
 }
 
-template <int VX>
-__global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
-    constexpr int H = step_halo_lanes(VX);
-    constexpr int TW = 64 * VX;
-    __shared__ float lds_p[2][TW];
-    __shared__ float lds_src[TW];
-    __shared__ float lds_out[TW];
+__device__ __forceinline__ void* slot_base(const ShotDev& s, const StepParams& prm) {
+    return static_cast<char*>(s.snap) + (size_t)prm.slot * s.snap_stride * (prm.snap_kind == SNAP_F32 ? 4 : 2);
+}
 
-    // Hardware hands block b to XCD b % 8; give each XCD a contiguous range of
-    // tasks so tiles that share halos share an L2.  gridDim.x is a multiple of 8.
+// Hardware hands block b to XCD b % 8; give each XCD a contiguous range of
+// tasks so tiles that share halos share an L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int task_index() {
     const int per = gridDim.x >> 3;
-    const int task = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Frame kernel: general update, one column per lane.
+// Lane l holds column x0 + l; lanes 0..H-1 and 64-H..63 only load halo.
+__global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
+    constexpr int H = frame_halo_lanes();
+    __shared__ float lds_p[2][64];
+    __shared__ float lds_src[64];
+    __shared__ float lds_out[64];
+
+    const int task = task_index();
     if (task >= prm.ntasks) return;
     const Task t = prm.tasks[task];
     const ShotDev& s = prm.shots[t.shot];
 
     const int lane = threadIdx.x;
     const int NY = prm.NY, dn = prm.dn, NX = s.NX, P = s.P;
-    const int x0 = t.xs - VX * H;           // first loaded column of the tile
-    const int xl = x0 + VX * lane;          // first column held by this lane
-    const bool ld_ok = (xl + VX) <= P;      // xl >= -4 always holds (front pad)
-    const bool compute_lane = lane >= H && lane < 64 - H;
+    const int x0 = t.xs - H;                 // first loaded column of the tile
+    const int c = x0 + lane;                 // this lane's column
+    const bool ld_ok = c >= t.xs - ACC && c < t.xe + ACC && c < P;
+    const bool in = lane >= H && lane < 64 - H && c >= t.xs && c < t.xe;
     const int cur = prm.k & 1;
     const float* __restrict__ pc = s.p[cur];     // time k-1
     float* __restrict__ pn = s.p[cur ^ 1];       // time k-2, overwritten with time k
@@ -133,100 +142,71 @@ __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
     float* __restrict__ chx = s.chx;
     float* __restrict__ chy = s.chy;
 
-    const int inj_dir = prm.mode;
-    const int* __restrict__ iptr = s.inj_ptr[inj_dir];
-    const int2* __restrict__ ient = s.inj_ent[inj_dir];
-    const float* __restrict__ itr_t = s.inj_tr[inj_dir] + (size_t)(prm.k - 2) * s.ntr[inj_dir];
-    float* __restrict__ rec_t = prm.record ? s.rec_tr + (size_t)(prm.k - 2) * s.nrec : nullptr;
+    const int mode = prm.mode;
+    const unsigned long long imask = mode ? t.inj_mask[1] : t.inj_mask[0];
+    const int ioff = mode ? t.inj_off[1] : t.inj_off[0];
+    const float* __restrict__ itr_t = (mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]);
+    const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
+    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
 
-    // column classification
-    int cx[VX], dxv[VX];
-    bool in[VX];
-#pragma unroll
-    for (int i = 0; i < VX; ++i) {
-        cx[i] = xl + i;
-        in[i] = compute_lane && cx[i] >= t.xs && cx[i] < t.xe;
-        dxv[i] = cx[i] < dn ? dn - cx[i] : (cx[i] >= NX - dn ? cx[i] - (NX - dn) + 1 : 0);
-    }
+    const int dxv = c < dn ? dn - c : (c >= NX - dn ? c - (NX - dn) + 1 : 0);
+    const bool model_col = c >= dn && c < dn + s.W;
 
     float snap_scale = 1.0f, snap_inv = 1.0f;
     void* snap_slot = nullptr;
     if (prm.snap_op != 0) {
-        snap_slot = static_cast<char*>(s.snap) +
-                    (size_t)prm.slot * s.snap_stride * (prm.snap_kind == SNAP_F32 ? 4 : 2);
+        snap_slot = slot_base(s, prm);
         if (prm.snap_kind != SNAP_F32 && s.snap_scale) {
             snap_scale = s.snap_scale[prm.slot];
             snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
         }
     }
 
-#pragma unroll
-    for (int i = 0; i < VX; ++i) lds_src[VX * lane + i] = 0.0f;
+    lds_src[lane] = 0.0f;
 
     // register window: rows yc-3 .. yc+3 of p at time k-1
-    float w[7][VX];
+    float w[7];
     int yc = clampi(t.ya, ACC, NY - 1 - ACC);
 #pragma unroll
-    for (int r = 0; r < 7; ++r) {
-        if (ld_ok) vload<VX>(w[r], pc + (size_t)(yc - 3 + r) * P + xl);
-        else vzero<VX>(w[r]);
-    }
-    float pmr[VX], c2r[VX];
-    if (ld_ok) {
-        vload<VX>(pmr, pn + (size_t)t.ya * P + xl);
-        vload<VX>(c2r, c2f + (size_t)t.ya * P + xl);
-    } else {
-        vzero<VX>(pmr);
-        vzero<VX>(c2r);
-    }
+    for (int r = 0; r < 7; ++r) w[r] = ld_ok ? pc[(size_t)(yc - 3 + r) * P + c] : 0.0f;
+    float pmr = in ? pn[(size_t)t.ya * P + c] : 0.0f;
+    float c2r = in ? c2f[(size_t)t.ya * P + c] : 0.0f;
 
     int buf = 0;
     for (int y = t.ya; y < t.yb; ++y) {
-        // prefetch the operands of the next row
         const int y1 = y + 1;
         const bool more = y1 < t.yb;
         const int yc1 = clampi(y1, ACC, NY - 1 - ACC);
         const bool shift = more && (yc1 != yc);
-        float npm[VX], nc2[VX], npw[VX];
-        vzero<VX>(npm);
-        vzero<VX>(nc2);
-        vzero<VX>(npw);
-        if (more && ld_ok) {
-            vload<VX>(npm, pn + (size_t)y1 * P + xl);
-            vload<VX>(nc2, c2f + (size_t)y1 * P + xl);
-            if (shift) vload<VX>(npw, pc + (size_t)(yc1 + 3) * P + xl);
+        float npm = 0.0f, nc2 = 0.0f, npw = 0.0f;
+        if (more && in) {
+            npm = pn[(size_t)y1 * P + c];
+            nc2 = c2f[(size_t)y1 * P + c];
         }
+        if (shift && ld_ok) npw = pc[(size_t)(yc1 + 3) * P + c];
 
         // p on row y (the window is centred on the clamped row yc)
         const int wr = y - yc + 3;
-        float prow[VX];
-#pragma unroll
-        for (int i = 0; i < VX; ++i) {
-            float v = w[3][i];
-            if (wr == 0) v = w[0][i];
-            else if (wr == 1) v = w[1][i];
-            else if (wr == 2) v = w[2][i];
-            else if (wr == 4) v = w[4][i];
-            else if (wr == 5) v = w[5][i];
-            else if (wr == 6) v = w[6][i];
-            prow[i] = v;
-        }
-#pragma unroll
-        for (int i = 0; i < VX; ++i) lds_p[buf][VX * lane + i] = prow[i];
+        float prow = w[3];
+        if (wr == 0) prow = w[0];
+        else if (wr == 1) prow = w[1];
+        else if (wr == 2) prow = w[2];
+        else if (wr == 4) prow = w[4];
+        else if (wr == 5) prow = w[5];
+        else if (wr == 6) prow = w[6];
+        lds_p[buf][lane] = prow;
 
-        // injection cells of this row and tile (model rows only)
+        const int ry = y - t.ya;
+        const bool has_inj = (imask >> ry) & 1ull;
         int ib = 0, ie = 0;
-        if (y >= dn && y < dn + prm.ny) {
-            const int ci = y * s.ntile + t.tile;
-            ib = iptr[ci];
-            ie = iptr[ci + 1];
-        }
-        const bool has_inj = ie > ib;
-        if (has_inj)
+        if (has_inj) {
+            ib = prm.rows[ioff + ry];
+            ie = prm.rows[ioff + ry + 1];
             for (int e = ib + lane; e < ie; e += 64) {
-                const int2 en = ient[e];
+                const int2 en = prm.ents[e];
                 lds_src[en.x - x0] = itr_t[en.y];
             }
+        }
         __syncthreads();
 
         const bool ytop = y < dn;
@@ -237,28 +217,23 @@ __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
         const int rbc = clampi(rb, ACC, dn - 1 - ACC);
         const bool model_row = y >= dn && y < dn + prm.ny;
 
-        float pp[VX];
-#pragma unroll
-        for (int i = 0; i < VX; ++i) {
-            pp[i] = 0.0f;
-            if (!in[i]) continue;
-            const int c = cx[i];
+        float pp = 0.0f;
+        if (in) {
             const int li = clampi(c, ACC, NX - 1 - ACC) - x0;
             const float* L = &lds_p[buf][li];
             const float d2x = dd2(L[-3], L[-2], L[-1], L[0], L[1], L[2], L[3]);
-            const float d2y = dd2(w[0][i], w[1][i], w[2][i], w[3][i], w[4][i], w[5][i], w[6][i]);
-            const float base = (-pmr[i]) + 2.0f * prow[i];
+            const float d2y = dd2(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+            const float base = (-pmr) + 2.0f * prow;
             const float lap = d2x + d2y;
-            const int dx = dxv[i];
-            if (dx == 0 && dyv == 0) {
+            if (dxv == 0 && dyv == 0) {
                 float q = lap;
                 if (has_inj) q = q + lds_src[c - x0];
-                pp[i] = base + q * c2r[i];
+                pp = base + q * c2r;
             } else {
-                const int m = dx > dyv ? dx : dyv;
+                const int m = dxv > dyv ? dxv : dyv;
                 const float A = prm.fA[m], B = prm.fB[m];
                 float propx = 0.0f, propy = 0.0f;
-                if (dx > 0) {
+                if (dxv > 0) {
                     const float d1x = dd1(L[-3], L[-2], L[-1], L[1], L[2], L[3]);
                     const int band = c < dn ? 0 : 1;
                     const int b = band == 0 ? c : c - (NX - dn);
@@ -273,7 +248,7 @@ __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
                     propx = dph + chn;
                 }
                 if (dyv > 0) {
-                    const float d1y = dd1(w[0][i], w[1][i], w[2][i], w[4][i], w[5][i], w[6][i]);
+                    const float d1y = dd1(w[0], w[1], w[2], w[4], w[5], w[6]);
                     const float* ph = phy_o + (size_t)(yband * dn) * P + c;
                     const float dph = dd1(ph[(size_t)(rbc - 3) * P], ph[(size_t)(rbc - 2) * P], ph[(size_t)(rbc - 1) * P],
                                           ph[(size_t)(rbc + 1) * P], ph[(size_t)(rbc + 2) * P], ph[(size_t)(rbc + 3) * P]);
@@ -284,75 +259,259 @@ __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
                     chy[idx] = chn;
                     propy = dph + chn;
                 }
-                if (dx > 0 && dyv > 0) pp[i] = (base + c2r[i] * (lap + propx)) + c2r[i] * propy;
-                else if (dx > 0) pp[i] = base + c2r[i] * (lap + propx);
-                else pp[i] = base + c2r[i] * (lap + propy);
+                if (dxv > 0 && dyv > 0) pp = (base + c2r * (lap + propx)) + c2r * propy;
+                else if (dxv > 0) pp = base + c2r * (lap + propx);
+                else pp = base + c2r * (lap + propy);
             }
-            pn[(size_t)y * P + c] = pp[i];
+            pn[(size_t)y * P + c] = pp;
 
-            if (prm.snap_op != 0 && model_row && c >= dn && c < dn + s.W) {
-                const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn);
+            if (prm.snap_op != 0 && model_row && model_col) {
+                const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
                 if (prm.snap_op == 1) {
-                    const float f = ((pp[i] - pmr[i]) * 0.5f) / prm.dt;
+                    const float f = ((pp - pmr) * 0.5f) / prm.dt;
                     snap_store(snap_slot, mi, f, prm.snap_kind, snap_inv);
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
-                    const float k2 = -(((0.5f * (pp[i] - pmr[i])) * f) * prm.ssf);
+                    const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
                     s.grad[mi] = s.grad[mi] + k2;
                 }
             }
         }
 
-        // receivers on this row and tile
-        if (rec_t) {
-            const int ci = y * s.ntile + t.tile;
-            const int rbeg = s.rec_ptr[ci], rend = s.rec_ptr[ci + 1];
-            if (rend > rbeg) {
-#pragma unroll
-                for (int i = 0; i < VX; ++i) lds_out[VX * lane + i] = pp[i];
-                __syncthreads();
-                for (int e = rbeg + lane; e < rend; e += 64) {
-                    const int2 en = s.rec_ent[e];
-                    rec_t[en.y] = lds_out[en.x - x0];
-                }
+        if ((rmask >> ry) & 1ull) {
+            lds_out[lane] = pp;
+            __syncthreads();
+            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            for (int e = rbeg + lane; e < rend; e += 64) {
+                const int2 en = prm.ents[e];
+                rec_t[en.y] = lds_out[en.x - x0];
             }
+            __syncthreads();
         }
         if (has_inj) {
             __syncthreads();
-            for (int e = ib + lane; e < ie; e += 64) lds_src[ient[e].x - x0] = 0.0f;
+            for (int e = ib + lane; e < ie; e += 64) lds_src[prm.ents[e].x - x0] = 0.0f;
             __syncthreads();
         }
 
-        // advance
-#pragma unroll
-        for (int i = 0; i < VX; ++i) {
-            pmr[i] = npm[i];
-            c2r[i] = nc2[i];
-        }
+        pmr = npm;
+        c2r = nc2;
         if (shift) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r)
-#pragma unroll
-                for (int i = 0; i < VX; ++i) w[r][i] = w[r + 1][i];
-#pragma unroll
-            for (int i = 0; i < VX; ++i) w[6][i] = npw[i];
+            for (int r = 0; r < 6; ++r) w[r] = w[r + 1];
+            w[6] = npw;
             yc = yc1;
         }
         buf ^= 1;
     }
 }
 
+// ---------------------------------------------------------------------------
+// Interior kernel: all cells of the task are model cells at least acc rows
+// and columns away from the frame, so no clamping and no CPML terms apply.
+// Lane l holds columns x0+4l .. x0+4l+3; lanes 0 and 63 only load halo.
+__global__ __launch_bounds__(64) void interior_kernel(const StepParams prm) {
+    __shared__ float4 lds_src[64];
+    __shared__ float4 lds_out[64];
+
+    const int task = task_index();
+    if (task >= prm.ntasks) return;
+    const Task t = prm.tasks[task];
+    const ShotDev& s = prm.shots[t.shot];
+
+    const int lane = threadIdx.x;
+    const int P = s.P, dn = prm.dn, Pm = prm.Pm;
+    const int x0 = t.xs - 4;
+    const int xl = x0 + 4 * lane;
+    const bool ld = xl <= t.xe + 2;                       // holds a column the tile reads
+    const bool cl = lane >= 1 && lane < 63 && xl < t.xe;  // holds a column the tile writes
+    const bool full = cl && xl + 4 <= t.xe;
+    const int cur = prm.k & 1;
+    const float* __restrict__ pc = s.p[cur];
+    float* __restrict__ pn = s.p[cur ^ 1];
+    const float* __restrict__ c2i = s.c2i;
+
+    const int mode = prm.mode;
+    const unsigned long long imask = mode ? t.inj_mask[1] : t.inj_mask[0];
+    const int ioff = mode ? t.inj_off[1] : t.inj_off[0];
+    const float* __restrict__ itr_t = (mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]);
+    const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
+    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
+    const int snap_op = prm.snap_op;
+    const int skind = prm.snap_kind;
+    float snap_scale = 1.0f, snap_inv = 1.0f;
+    void* snap_slot = nullptr;
+    if (snap_op != 0) {
+        snap_slot = slot_base(s, prm);
+        if (skind != SNAP_F32 && s.snap_scale) {
+            snap_scale = s.snap_scale[prm.slot];
+            snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
+        }
+    }
+    const int mcol = xl - dn + s.moff;      // model storage column of this lane's first cell
+    const float* lanes_p = pc + xl;
+    float* lanes_pn = pn + xl;
+    const float* lanes_c2 = c2i + xl;
+
+    lds_src[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    // slots S[(j+0..6)&7] hold rows y-3..y+3 at step j; S[(j+7)&7] receives row y+4
+    float4 S[8];
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) S[r] = ld ? ld4(lanes_p + (size_t)(t.ya - 3 + r) * P) : z4;
+    S[7] = z4;
+    float4 PM[2], C2[2];
+    PM[0] = cl ? ld4(lanes_pn + (size_t)t.ya * P) : z4;
+    C2[0] = cl ? ld4(lanes_c2 + (size_t)(t.ya - dn) * Pm) : z4;
+    PM[1] = z4;
+    C2[1] = z4;
+
+    // One row; J is the row's position in the 8-row rotation of S.  Returns
+    // false past the task's last row.
+    auto row = [&](int y0, auto Jc) -> bool {
+        constexpr int j = decltype(Jc)::value;
+        const int y = y0 + j;
+        if (y >= t.yb) return false;
+        // prefetch the next row's operands
+        if (y + 1 < t.yb) {
+            if (ld) S[(j + 7) & 7] = ld4(lanes_p + (size_t)(y + 4) * P);
+            if (cl) {
+                PM[(j + 1) & 1] = ld4(lanes_pn + (size_t)(y + 1) * P);
+                C2[(j + 1) & 1] = ld4(lanes_c2 + (size_t)(y + 1 - dn) * Pm);
+            }
+        }
+        const float4 m3 = S[j & 7], m2 = S[(j + 1) & 7], m1 = S[(j + 2) & 7], c = S[(j + 3) & 7];
+        const float4 p1 = S[(j + 4) & 7], p2 = S[(j + 5) & 7], p3 = S[(j + 6) & 7];
+        const float4 pm = PM[j & 1], cc = C2[j & 1];
+
+        // x neighbours of the row (all lanes active here)
+        const float l1 = from_left(c.y), l2 = from_left(c.z), l3 = from_left(c.w);
+        const float r0 = from_right(c.x), r1 = from_right(c.y), r2 = from_right(c.z);
+
+        const int ry = y - t.ya;
+        const bool has_inj = (imask >> ry) & 1ull;
+        int ib = 0, ie = 0;
+        float4 src = z4;
+        if (has_inj) {
+            ib = prm.rows[ioff + ry];
+            ie = prm.rows[ioff + ry + 1];
+            float* ls = reinterpret_cast<float*>(lds_src);
+            for (int e = ib + lane; e < ie; e += 64) {
+                const int2 en = prm.ents[e];
+                ls[en.x - x0] = itr_t[en.y];
+            }
+            __syncthreads();
+            src = lds_src[lane];
+            __syncthreads();
+            for (int e = ib + lane; e < ie; e += 64) ls[prm.ents[e].x - x0] = 0.0f;
+            __syncthreads();
+        }
+
+        float4 pp;
+        {
+            float lap0 = dd2(l1, l2, l3, c.x, c.y, c.z, c.w) + dd2(m3.x, m2.x, m1.x, c.x, p1.x, p2.x, p3.x);
+            float lap1 = dd2(l2, l3, c.x, c.y, c.z, c.w, r0) + dd2(m3.y, m2.y, m1.y, c.y, p1.y, p2.y, p3.y);
+            float lap2 = dd2(l3, c.x, c.y, c.z, c.w, r0, r1) + dd2(m3.z, m2.z, m1.z, c.z, p1.z, p2.z, p3.z);
+            float lap3 = dd2(c.x, c.y, c.z, c.w, r0, r1, r2) + dd2(m3.w, m2.w, m1.w, c.w, p1.w, p2.w, p3.w);
+            if (has_inj) {
+                lap0 = lap0 + src.x;
+                lap1 = lap1 + src.y;
+                lap2 = lap2 + src.z;
+                lap3 = lap3 + src.w;
+            }
+            pp.x = ((-pm.x) + 2.0f * c.x) + lap0 * cc.x;
+            pp.y = ((-pm.y) + 2.0f * c.y) + lap1 * cc.y;
+            pp.z = ((-pm.z) + 2.0f * c.z) + lap2 * cc.z;
+            pp.w = ((-pm.w) + 2.0f * c.w) + lap3 * cc.w;
+        }
+
+        float* dst = lanes_pn + (size_t)y * P;
+        if (full) {
+            st4(dst, pp);
+        } else if (cl) {
+            dst[0] = pp.x;
+            if (xl + 1 < t.xe) dst[1] = pp.y;
+            if (xl + 2 < t.xe) dst[2] = pp.z;
+        }
+
+        if (snap_op != 0 && cl) {
+            const size_t mi = (size_t)(y - dn) * s.Ws + mcol;
+            const float ppv[4] = {pp.x, pp.y, pp.z, pp.w};
+            const float pmv[4] = {pm.x, pm.y, pm.z, pm.w};
+            if (snap_op == 1) {
+                if (full && skind == SNAP_F32) {
+                    float4 f;
+                    f.x = ((pp.x - pm.x) * 0.5f) / prm.dt;
+                    f.y = ((pp.y - pm.y) * 0.5f) / prm.dt;
+                    f.z = ((pp.z - pm.z) * 0.5f) / prm.dt;
+                    f.w = ((pp.w - pm.w) * 0.5f) / prm.dt;
+                    st4(static_cast<float*>(snap_slot) + mi, f);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (xl + i < t.xe)
+                            snap_store(snap_slot, mi + i, ((ppv[i] - pmv[i]) * 0.5f) / prm.dt, skind, snap_inv);
+                }
+            } else {
+                float* g = s.grad + mi;
+                if (full && skind == SNAP_F32) {
+                    const float4 f = ld4(static_cast<const float*>(snap_slot) + mi);
+                    float4 gv = ld4(g);
+                    gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
+                    gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
+                    gv.z = gv.z + (-(((0.5f * (pp.z - pm.z)) * f.z) * prm.ssf));
+                    gv.w = gv.w + (-(((0.5f * (pp.w - pm.w)) * f.w) * prm.ssf));
+                    st4(g, gv);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (xl + i < t.xe) {
+                            const float f = snap_load(snap_slot, mi + i, skind, snap_scale);
+                            g[i] = g[i] + (-(((0.5f * (ppv[i] - pmv[i])) * f) * prm.ssf));
+                        }
+                }
+            }
+        }
+
+        if ((rmask >> ry) & 1ull) {
+            lds_out[lane] = pp;
+            __syncthreads();
+            const float* lo = reinterpret_cast<const float*>(lds_out);
+            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            for (int e = rbeg + lane; e < rend; e += 64) {
+                const int2 en = prm.ents[e];
+                rec_t[en.y] = lo[en.x - x0];
+            }
+            __syncthreads();
+        }
+        return true;
+    };
+    for (int y0 = t.ya; y0 < t.yb; y0 += 8) {
+        if (!row(y0, std::integral_constant<int, 0>{})) break;
+        if (!row(y0, std::integral_constant<int, 1>{})) break;
+        if (!row(y0, std::integral_constant<int, 2>{})) break;
+        if (!row(y0, std::integral_constant<int, 3>{})) break;
+        if (!row(y0, std::integral_constant<int, 4>{})) break;
+        if (!row(y0, std::integral_constant<int, 5>{})) break;
+        if (!row(y0, std::integral_constant<int, 6>{})) break;
+        if (!row(y0, std::integral_constant<int, 7>{})) break;
+    }
+}
+
 }  // namespace
 
-hipError_t launch_step(int vx, const StepParams& prm, hipStream_t stream) {
+hipError_t launch_frame(const StepParams& prm, hipStream_t stream) {
     const int nb = ((prm.ntasks + 7) / 8) * 8;
     if (nb == 0) return hipSuccess;
-    switch (vx) {
-        case 1: hipLaunchKernelGGL(step_kernel<1>, dim3(nb), dim3(64), 0, stream, prm); break;
-        case 2: hipLaunchKernelGGL(step_kernel<2>, dim3(nb), dim3(64), 0, stream, prm); break;
-        case 4: hipLaunchKernelGGL(step_kernel<4>, dim3(nb), dim3(64), 0, stream, prm); break;
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(frame_kernel, dim3(nb), dim3(64), 0, stream, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_interior(const StepParams& prm, hipStream_t stream) {
+    const int nb = ((prm.ntasks + 7) / 8) * 8;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(interior_kernel, dim3(nb), dim3(64), 0, stream, prm);
     return hipGetLastError();
 }
 

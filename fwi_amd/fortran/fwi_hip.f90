@@ -1,0 +1,116 @@
+! ISO_C_BINDING interface to the batched context API of libfwi_hip.so
+! (include/fwi.h).  A Fortran inversion driver uses it to keep all shots of
+! a gradient evaluation resident on the GPU:
+!
+!   use fwi_hip
+!   type(fwi_config_t) :: cfg
+!   type(c_ptr) :: ctx
+!   ierr = fwi_create(cfg, ctx)
+!   ierr = fwi_set_shots(ctx, nshots, shots)      ! windows + acquisition
+!   ierr = fwi_set_model(ctx, vp)                  ! vp(ny,nx)
+!   ierr = fwi_set_sources(ctx, S) ; ierr = fwi_set_observed(ctx, D)
+!   ierr = fwi_gradient(ctx, 1d0/nshots, water_rows, misfit)
+!   ierr = fwi_get_gradient(ctx, grad)             ! grad(ny,nx), REAL(8)
+!   call fwi_destroy(ctx)
+!
+! The drop-in replacement of solver_acoustic_SS needs no module: the library
+! exports `solver_acoustic_ss_` with the reference's implicit-interface
+! calling convention.
+module fwi_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: FWI_SNAP_F32 = 0, FWI_SNAP_BF16 = 1, FWI_SNAP_F16 = 2
+
+  type, bind(C) :: fwi_config_t
+     integer(c_int) :: ny, nx
+     real(c_float)  :: d, dt
+     integer(c_int) :: nt, store_snap, dpml, free_surface, snap_kind, rows_per_task
+  end type fwi_config_t
+
+  type, bind(C) :: fwi_shot_t
+     integer(c_int) :: x0, w, nsrc
+     type(c_ptr)    :: src_x, src_y
+     integer(c_int) :: nrec
+     type(c_ptr)    :: rec_x, rec_y
+  end type fwi_shot_t
+
+  type, bind(C) :: fwi_solver_opts_t
+     integer(c_int) :: dpml, free_surface
+  end type fwi_solver_opts_t
+
+  interface
+     integer(c_int) function fwi_create(cfg, ctx) bind(C, name="fwi_create")
+       import :: c_int, c_ptr, fwi_config_t
+       type(fwi_config_t), intent(in) :: cfg
+       type(c_ptr), intent(out) :: ctx
+     end function fwi_create
+
+     subroutine fwi_destroy(ctx) bind(C, name="fwi_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine fwi_destroy
+
+     integer(c_int) function fwi_set_shots(ctx, nshots, shots) bind(C, name="fwi_set_shots")
+       import :: c_int, c_ptr, fwi_shot_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nshots
+       type(fwi_shot_t), intent(in) :: shots(*)
+     end function fwi_set_shots
+
+     integer(c_int) function fwi_set_model(ctx, vp) bind(C, name="fwi_set_model")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: vp(*)
+     end function fwi_set_model
+
+     integer(c_int) function fwi_set_sources(ctx, s) bind(C, name="fwi_set_sources")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: s(*)
+     end function fwi_set_sources
+
+     integer(c_int) function fwi_set_observed(ctx, d) bind(C, name="fwi_set_observed")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(in) :: d(*)
+     end function fwi_set_observed
+
+     integer(c_int) function fwi_forward(ctx, store_snapshots) bind(C, name="fwi_forward")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: store_snapshots
+     end function fwi_forward
+
+     integer(c_int) function fwi_get_synthetic(ctx, out) bind(C, name="fwi_get_synthetic")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: out(*)
+     end function fwi_get_synthetic
+
+     integer(c_int) function fwi_gradient(ctx, scale, water_rows, misfit) bind(C, name="fwi_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: scale
+       integer(c_int), intent(in) :: water_rows(*)
+       real(c_double), intent(out) :: misfit
+     end function fwi_gradient
+
+     integer(c_int) function fwi_get_gradient(ctx, grad) bind(C, name="fwi_get_gradient")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: grad(*)
+     end function fwi_get_gradient
+
+     integer(c_int) function fwi_solver_acoustic(opts, iprop, nsou, nrec, nxsou, nysou, nxrec, nyrec, s, sadj, &
+          d, dt, ny, nx, ctmp, store_snap, nt_store, nt, kernel, fields, rec_point) bind(C, name="fwi_solver_acoustic")
+       import :: c_int, c_float, fwi_solver_opts_t
+       type(fwi_solver_opts_t), intent(in) :: opts
+       integer(c_int), value :: iprop, nsou, nrec, ny, nx, store_snap, nt_store, nt
+       integer(c_int), intent(in) :: nxsou(*), nysou(*), nxrec(*), nyrec(*)
+       real(c_float), intent(in) :: s(*), sadj(*), ctmp(*)
+       real(c_float), value :: d, dt
+       real(c_float), intent(inout) :: kernel(*), fields(*), rec_point(*)
+     end function fwi_solver_acoustic
+  end interface
+end module fwi_hip

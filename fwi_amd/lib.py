@@ -31,7 +31,7 @@ class SolverOpts(C.Structure):
 class Config(C.Structure):
     _fields_ = [("ny", C.c_int), ("nx", C.c_int), ("d", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
                 ("store_snap", C.c_int), ("dpml", C.c_int), ("free_surface", C.c_int), ("snap_kind", C.c_int),
-                ("vx", C.c_int), ("rows_per_task", C.c_int)]
+                ("rows_per_task", C.c_int)]
 
 
 class Shot(C.Structure):
@@ -148,9 +148,9 @@ def solver_forward_simple(nxSou, nySou, nxRec, nyRec, S, d, dt, ctmp, nt):
 class Context:
     """Batched shots resident on one GPU (the fwi_ctx C API)."""
 
-    def __init__(self, ny, nx, d, dt, nt, store_snap=10, dpml=20, free_surface=1, snap_kind=SNAP_F32, vx=0,
+    def __init__(self, ny, nx, d, dt, nt, store_snap=10, dpml=20, free_surface=1, snap_kind=SNAP_F32,
                  rows_per_task=0):
-        self.cfg = Config(ny, nx, d, dt, nt, store_snap, dpml, free_surface, snap_kind, vx, rows_per_task)
+        self.cfg = Config(ny, nx, d, dt, nt, store_snap, dpml, free_surface, snap_kind, rows_per_task)
         h = _P()
         _check(lib().fwi_create(C.byref(self.cfg), C.byref(h)), "fwi_create")
         self.h = h

@@ -94,8 +94,7 @@ typedef struct fwi_config {
     int dpml;              /* CPML thickness (reference: 20) */
     int free_surface;      /* as in fwi_solver_opts */
     int snap_kind;         /* FWI_SNAP_F32 / _BF16 / _F16 */
-    int vx;                /* kernel columns per lane: 1, 2 or 4 (0 = default) */
-    int rows_per_task;     /* rows per work unit (0 = default) */
+    int rows_per_task;     /* rows per kernel work unit, 1..64 (0 = default 32) */
 } fwi_config;
 
 /* One shot: an aperture window of the model plus its acquisition, positions

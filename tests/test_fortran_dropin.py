@@ -1,0 +1,85 @@
+"""Drop-in proof: one Fortran program (tests/fortran/dropin_host.f90) calling
+solver_acoustic_SS through an implicit interface, linked once against the
+reference's compiled solver and once against libfwi_hip.so."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import refbind as rb
+from fwi_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "fortran", "_build")
+HIP_HOST = os.path.join(BUILD, "dropin_host_hip")
+REF_HOST = os.path.join(BUILD, "dropin_host_ref")
+
+
+def write_input(path, seed=0, ny=40, nx=60, nt=200, ss=10):
+    c, S, nxs, nys, nxr, nyr = synth.small_case(seed, ny, nx, nt)
+    with open(path, "wb") as f:
+        np.array([ny, nx, nt, len(nxs), len(nxr), ss], np.int32).tofile(f)
+        np.array([25.0, 0.002], np.float32).tofile(f)
+        for a in (nxs, nys, nxr, nyr):
+            np.asarray(a, np.int32).tofile(f)
+        np.asfortranarray(S, np.float32).ravel(order="F").tofile(f)
+        np.asfortranarray(c, np.float32).ravel(order="F").tofile(f)
+    return c, S, nxs, nys, nxr, nyr, ny, nx, nt, ss
+
+
+def read_output(path, ny, nx, nt, nr, ss):
+    raw = np.fromfile(path, np.float32)
+    n1, n2 = nt * nr, ny * nx
+    rec = raw[:n1].reshape((nt, nr), order="F")
+    ker = raw[n1:n1 + n2].reshape((ny, nx), order="F")
+    fields = raw[n1 + n2:].reshape((ny, nx * (nt // ss + 1)), order="F")
+    return rec, ker, fields
+
+
+def oracle_expected(c, S, nxs, nys, nxr, nyr, nt, ss):
+    d, dt = np.float32(25.0), np.float32(0.002)
+    F = rb.oracle_solver("f32", 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nt // ss, nt, free_surface=1)
+    adj = -F.rec[::-1].copy()
+    A = rb.oracle_solver("f32", 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nt // ss, nt, fields=F.fields,
+                         free_surface=1)
+    return F.rec, A.kernel, F.fields
+
+
+def test_fortran_hosts_are_built():
+    if shutil.which("make") is None or not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
+        pytest.skip("no flang here")
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "fortran")], check=True, capture_output=True)
+    assert os.path.exists(HIP_HOST)
+    assert os.path.exists(os.path.join(BUILD, "fwi_hip.o"))      # the ISO_C_BINDING module compiles
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HOST), reason="reference-linked host not built")
+def test_reference_linked_host_matches_oracle(tmp_path):
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    c, S, nxs, nys, nxr, nyr, ny, nx, nt, ss = write_input(inp)
+    subprocess.run([REF_HOST, inp, out], check=True, timeout=120)
+    rec, ker, fields = read_output(out, ny, nx, nt, len(nxr), ss)
+    erec, eker, efields = oracle_expected(c, S, nxs, nys, nxr, nyr, nt, ss)
+    assert np.array_equal(rec, erec) and np.array_equal(ker, eker) and np.array_equal(fields, efields)
+
+
+@pytest.mark.gpu
+def test_hip_linked_host_matches_reference(tmp_path):
+    if not os.path.exists(HIP_HOST):
+        pytest.fail("tests/fortran/_build/dropin_host_hip is not built")
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    c, S, nxs, nys, nxr, nyr, ny, nx, nt, ss = write_input(inp, seed=1)
+    r = subprocess.run([HIP_HOST, inp, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec, ker, fields = read_output(out, ny, nx, nt, len(nxr), ss)
+    erec, eker, efields = oracle_expected(c, S, nxs, nys, nxr, nyr, nt, ss)
+    assert np.abs(erec).max() > 0
+    assert np.array_equal(rec, erec), np.abs(rec - erec).max()
+    assert np.array_equal(fields, efields)
+    assert np.array_equal(ker, eker), np.abs(ker - eker).max()
+    if os.path.exists(REF_HOST):
+        out2 = str(tmp_path / "out_ref.bin")
+        subprocess.run([REF_HOST, inp, out2], check=True, timeout=120)
+        assert open(out, "rb").read() == open(out2, "rb").read()

@@ -116,8 +116,8 @@ def test_duplicates_and_pml_positions():
     assert np.array_equal(k, O2.kernel)
 
 
-@pytest.mark.parametrize("vx,rows", [(1, 16), (2, 64), (4, 64), (4, 7), (4, 1000)])
-def test_batched_windows_vs_oracle(vx, rows):
+@pytest.mark.parametrize("rows", [1, 7, 16, 32, 64])
+def test_batched_windows_vs_oracle(rows):
     """Several shots with uneven windows in one context; each shot equals the
     reference solver run on its window sub-model."""
     rng = np.random.default_rng(7)
@@ -132,7 +132,7 @@ def test_batched_windows_vs_oracle(vx, rows):
                           rec_x=rng.integers(1, w + 1, nr).tolist(), rec_y=rng.integers(1, ny + 1, nr).tolist()))
         wav.append(synth.ricker(float(rng.uniform(8, 14)), nt, 0.002)[:, None])
     d, dt = np.float32(20.0), np.float32(0.002)
-    ctx = fwi_amd.Context(ny, nx, d, dt, nt, ss, dpml=20, free_surface=0, vx=vx, rows_per_task=rows)
+    ctx = fwi_amd.Context(ny, nx, d, dt, nt, ss, dpml=20, free_surface=0, rows_per_task=rows)
     ctx.set_shots(shots)
     ctx.set_model(vp)
     ctx.set_sources(wav)
