@@ -117,9 +117,8 @@ struct fwi_ctx {
     int nrec_total = 0, nsrc_total = 0;
     Arena state, stat, traces, snaps, grads, misc;
     fwi::ShotDev* d_desc = nullptr;
-    fwi::Task* d_ftasks = nullptr;        // frame tasks
-    fwi::Task* d_itasks = nullptr;        // interior tasks
-    int nftasks = 0, nitasks = 0;
+    fwi::Task* d_tasks = nullptr;         // frame and interior tasks, grouped per XCD
+    int ntasks = 0;
     int* d_rows = nullptr;
     int2* d_ents = nullptr;
     float* d_c2s = nullptr;               // four shifted copies of the model c2 (interior loads)
@@ -218,7 +217,8 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     const int dn = c->dn, NY = c->NY, ny = cf.ny;
     c->shots.assign(nshots, Shot());
     c->nrec_total = c->nsrc_total = 0;
-    std::vector<fwi::Task> ftasks, itasks;
+    std::vector<std::vector<fwi::Task>> ftasks(nshots), itasks(nshots);
+    std::vector<long long> work(nshots, 0);
     std::vector<int> rows;
     std::vector<int2> ents;
     const int R = std::min(c->rows, fwi::MAX_TASK_ROWS);
@@ -258,10 +258,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         }
         for (size_t j = 0; j < s.rx.size(); ++j) rec.push_back(Cell{dn + s.ry[j] - 1, dn + s.rx[j] - 1, (int)j});
 
-        auto add_task = [&](std::vector<fwi::Task>& list, int xs, int xe, int ya, int yb) {
+        auto add_task = [&](std::vector<fwi::Task>& list, int kind, int xs, int xe, int ya, int yb) {
             fwi::Task t;
             std::memset(&t, 0, sizeof(t));
-            t.shot = i; t.xs = xs; t.xe = xe; t.ya = ya; t.yb = yb;
+            t.kind = kind; t.shot = i; t.xs = xs; t.xe = xe; t.ya = ya; t.yb = yb;
             for (int dir = 0; dir < 2; ++dir) attach_cells(t, inj[dir], rows, ents, t.inj_off[dir], t.inj_mask[dir]);
             attach_cells(t, rec, rows, ents, t.rec_off, t.rec_mask);
             list.push_back(t);
@@ -269,19 +269,27 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         // interior: model rows, 4-aligned columns [XL, XR) away from the frame
         const int XR = s.NX - dn;
         const int XL = std::min((int)align_up(dn, 4), XR);
+        const int IT = fwi::TASK_INTERIOR, FT = fwi::TASK_FRAME;
         for (auto& yr : row_range(dn, NY - dn, R))
-            for (auto& xr : tile_range(XL, XR, fwi::interior_tile_cols(), -1)) add_task(itasks, xr.first, xr.second, yr.first, yr.second);
-        // frame: left and right bands over the model rows, bottom (and top) bands over all columns
+            for (auto& xr : tile_range(XL, XR, fwi::interior_tile_cols(), -1))
+                add_task(itasks[i], IT, xr.first, xr.second, yr.first, yr.second);
+        // frame: the left band [0, XL) and right band [XR, NX) each form one
+        // tile; over the model rows they are the whole task, in the bottom
+        // (and top) band rows the columns between them are tiled too
         const int FW = fwi::frame_tile_cols();
+        std::vector<std::pair<int, int>> band_cols = {{0, XL}};
+        for (auto& xr : tile_range(XL, XR, FW, -1)) band_cols.push_back(xr);
+        band_cols.push_back({XR, s.NX});
         for (auto& yr : row_range(dn, NY - dn, R)) {
-            for (auto& xr : tile_range(0, XL, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
-            for (auto& xr : tile_range(XR, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+            add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
+            add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
         }
         for (auto& yr : row_range(NY - dn, NY, R))
-            for (auto& xr : tile_range(0, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+            for (auto& xr : band_cols) add_task(ftasks[i], FT, xr.first, xr.second, yr.first, yr.second);
         if (!cf.free_surface)
             for (auto& yr : row_range(0, dn, R))
-                for (auto& xr : tile_range(0, s.NX, FW, s.NX)) add_task(ftasks, xr.first, xr.second, yr.first, yr.second);
+                for (auto& xr : band_cols) add_task(ftasks[i], FT, xr.first, xr.second, yr.first, yr.second);
+        work[i] = (long long)(NY - (cf.free_surface ? dn : 0)) * s.NX;
 
         s.roff = c->nrec_total;
         s.toff = (size_t)c->nrec_total * cf.nt;
@@ -320,15 +328,39 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         s.o_obs = o_obs + s.toff * sizeof(float);
         s.o_adj = o_adj + s.toff * sizeof(float);
     }
-    c->nftasks = (int)ftasks.size();
-    c->nitasks = (int)itasks.size();
+    // XCD groups: block b runs on XCD b % 8 and takes task (b % 8) * per + b / 8,
+    // so group g is a contiguous range of `per` tasks.  Shots go to groups
+    // largest first onto the least-loaded group; inside a group the (slower,
+    // latency-bound) frame tasks come first.
+    std::vector<int> order(nshots);
+    for (int i = 0; i < nshots; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<std::vector<int>> groups(8);
+    std::vector<long long> load(8, 0);
+    for (int i : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups[g].push_back(i);
+        load[g] += work[i];
+    }
+    std::vector<std::vector<fwi::Task>> gt(8);
+    size_t per = 0;
+    for (int g = 0; g < 8; ++g) {
+        std::sort(groups[g].begin(), groups[g].end());
+        for (int i : groups[g]) gt[g].insert(gt[g].end(), ftasks[i].begin(), ftasks[i].end());
+        for (int i : groups[g]) gt[g].insert(gt[g].end(), itasks[i].begin(), itasks[i].end());
+        per = std::max(per, gt[g].size());
+    }
+    std::vector<fwi::Task> tasks(8 * per);
+    for (auto& t : tasks) { std::memset(&t, 0, sizeof(t)); t.kind = fwi::TASK_IDLE; }
+    for (int g = 0; g < 8; ++g)
+        for (size_t j = 0; j < gt[g].size(); ++j) tasks[g * per + j] = gt[g][j];
+    c->ntasks = (int)tasks.size();
 
     // misc arena: descriptors, tasks, misfit scratch, gradient
     const int nblk = 64;
     c->nblk_misfit = nblk;
     const size_t o_desc = c->misc.add(nshots * sizeof(fwi::ShotDev));
-    const size_t o_ft = c->misc.add(std::max<size_t>(1, ftasks.size()) * sizeof(fwi::Task));
-    const size_t o_it = c->misc.add(std::max<size_t>(1, itasks.size()) * sizeof(fwi::Task));
+    const size_t o_tasks = c->misc.add(std::max<size_t>(1, tasks.size()) * sizeof(fwi::Task));
     const size_t o_vp = c->misc.add((size_t)cf.ny * cf.nx * sizeof(float));
     const size_t o_c2sh = c->misc.add(nshots * sizeof(fwi::C2Shot));
     const size_t o_rmap = c->misc.add(std::max(1, c->nrec_total) * sizeof(fwi::RecMap));
@@ -349,8 +381,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     HIPCHK(hipMemset(c->stat.base, 0, c->stat.size));
 
     c->d_desc = c->misc.at<fwi::ShotDev>(o_desc);
-    c->d_ftasks = c->misc.at<fwi::Task>(o_ft);
-    c->d_itasks = c->misc.at<fwi::Task>(o_it);
+    c->d_tasks = c->misc.at<fwi::Task>(o_tasks);
     c->d_vp = c->misc.at<float>(o_vp);
     c->d_c2shots = c->misc.at<fwi::C2Shot>(o_c2sh);
     c->d_recmap = c->misc.at<fwi::RecMap>(o_rmap);
@@ -406,8 +437,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         gs[si] = fwi::GradShot{d.grad, s.x0, s.W, s.Ws, s.moff};
     }
     HIPCHK(hipMemcpy(c->d_desc, desc.data(), nshots * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
-    if (!ftasks.empty()) HIPCHK(hipMemcpy(c->d_ftasks, ftasks.data(), ftasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
-    if (!itasks.empty()) HIPCHK(hipMemcpy(c->d_itasks, itasks.data(), itasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
+    if (!tasks.empty()) HIPCHK(hipMemcpy(c->d_tasks, tasks.data(), tasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPCHK(hipMemcpy(c->d_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ents.empty()) HIPCHK(hipMemcpy(c->d_ents, ents.data(), ents.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_c2shots, c2s.data(), nshots * sizeof(fwi::C2Shot), hipMemcpyHostToDevice));
@@ -430,6 +460,8 @@ int run(fwi_ctx* c, int mode, bool store) {
     fwi::StepParams prm;
     std::memset(&prm, 0, sizeof(prm));
     prm.shots = c->d_desc;
+    prm.tasks = c->d_tasks;
+    prm.ntasks = c->ntasks;
     prm.rows = c->d_rows;
     prm.ents = c->d_ents;
     prm.Pm = c->Pm;
@@ -471,12 +503,7 @@ int run(fwi_ctx* c, int mode, bool store) {
                 prm.slot = slot;
             }
         }
-        prm.tasks = c->d_ftasks;
-        prm.ntasks = c->nftasks;
-        HIPCHK(fwi::launch_frame(prm, c->stream));
-        prm.tasks = c->d_itasks;
-        prm.ntasks = c->nitasks;
-        HIPCHK(fwi::launch_interior(prm, c->stream));
+        HIPCHK(fwi::launch_step(prm, c->stream));
     }
     if (c->timing) {
         HIPCHK(hipEventRecord(ev1, c->stream));

@@ -7,7 +7,7 @@
 namespace fwi {
 
 constexpr int ACC = 3;        // stencil radius (solver_acoustic_SS.f90:91, acc=3)
-constexpr int MAXDN = 96;     // largest CPML frame dn = dpml + 2*acc supported
+constexpr int MAXDN = 58;     // largest CPML frame dn = dpml + 2*acc supported (one band per frame tile)
 constexpr int MAX_TASK_ROWS = 64;   // rows per task (bitmask width)
 
 // Snapshot storage precisions.
@@ -48,12 +48,15 @@ struct ShotDev {
 // per row: rows[off + (y - ya)] .. rows[off + (y - ya) + 1] index the entry
 // array; the masks flag the rows that have entries.
 struct Task {
+    int kind;                     // TASK_FRAME, TASK_INTERIOR or TASK_IDLE
     int shot, xs, xe, ya, yb;
     int inj_off[2];               // [0] forward sources, [1] adjoint (receiver cells)
-    int rec_off;
+    int rec_off, pad0;
     unsigned long long inj_mask[2];
     unsigned long long rec_mask;
 };
+
+enum TaskKind : int { TASK_IDLE = -1, TASK_FRAME = 0, TASK_INTERIOR = 1 };
 
 // Per-launch parameters of the time-step kernels.
 struct StepParams {
@@ -84,7 +87,8 @@ constexpr int frame_tile_cols() { return (64 - 2 * frame_halo_lanes()) * FRAME_V
 constexpr int INTERIOR_VX = 4;                                      // interior kernel: 4 columns per lane
 constexpr int interior_tile_cols() { return (64 - 2) * INTERIOR_VX; }   // lanes 0 and 63 only load halo
 
-hipError_t launch_frame(const StepParams& prm, hipStream_t stream);
-hipError_t launch_interior(const StepParams& prm, hipStream_t stream);
+// One launch advances every shot by one step; ntasks must be a multiple of
+// 8 (tasks grouped per XCD, padded with TASK_IDLE).
+hipError_t launch_step(const StepParams& prm, hipStream_t stream);
 
 }  // namespace fwi

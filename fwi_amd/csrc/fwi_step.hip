@@ -111,18 +111,31 @@ __device__ __forceinline__ int task_index() {
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-// ---------------------------------------------------------------------------
-// Frame kernel: general update, one column per lane.
-// Lane l holds column x0 + l; lanes 0..H-1 and 64-H..63 only load halo.
-__global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
-    constexpr int H = frame_halo_lanes();
-    __shared__ float lds_p[2][64];
-    __shared__ float lds_src[64];
-    __shared__ float lds_out[64];
+// LDS of one wavefront.
+struct FrameLds {
+    float p[2][64];       // current row of p (double-buffered by row parity)
+    float phx[2][64];     // current row of phi_x (x-band lanes)
+    float src[64];        // injected samples of the row
+    float out[64];        // updated row, for receiver sampling
+};
+struct InteriorLds {
+    float4 src[64];
+    float4 out[64];
+};
+union StepLds {
+    FrameLds f;
+    InteriorLds i;
+};
 
-    const int task = task_index();
-    if (task >= prm.ntasks) return;
-    const Task t = prm.tasks[task];
+// ---------------------------------------------------------------------------
+// Frame task: general update, one column per lane.  Lane l holds column
+// x0 + l; lanes 0..2 and 61..63 only load halo.  Every operand of row y+1 is
+// requested before row y is computed.  A task lies either entirely inside a
+// y band (all its cells carry y-CPML terms) or entirely outside; an x band
+// always lies inside a single tile, so phi_x of the clamped band columns is
+// exchanged through LDS.
+__device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t, FrameLds& L) {
+    constexpr int H = frame_halo_lanes();
     const ShotDev& s = prm.shots[t.shot];
 
     const int lane = threadIdx.x;
@@ -149,8 +162,18 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
     const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
     float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
 
+    // x band of this lane
     const int dxv = c < dn ? dn - c : (c >= NX - dn ? c - (NX - dn) + 1 : 0);
+    const bool xpml = in && dxv > 0;
+    const int band = c < dn ? 0 : 1;
+    const int bcol = band == 0 ? c : c - (NX - dn);                  // column inside the band
+    const int bclamp = clampi(bcol, ACC, dn - 1 - ACC);
+    const int lb = (band == 0 ? bclamp : bclamp + NX - dn) - x0;    // lane of the clamped band column
     const bool model_col = c >= dn && c < dn + s.W;
+    // y band of the task (uniform)
+    const bool ytask = t.ya < dn || t.ya >= NY - dn;
+    const int yband = t.ya < dn ? 0 : 1;
+    const int yb0 = yband == 0 ? 0 : NY - dn;                       // first row of the band
 
     float snap_scale = 1.0f, snap_inv = 1.0f;
     void* snap_slot = nullptr;
@@ -162,7 +185,7 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
         }
     }
 
-    lds_src[lane] = 0.0f;
+    L.src[lane] = 0.0f;
 
     // register window: rows yc-3 .. yc+3 of p at time k-1
     float w[7];
@@ -171,17 +194,46 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
     for (int r = 0; r < 7; ++r) w[r] = ld_ok ? pc[(size_t)(yc - 3 + r) * P + c] : 0.0f;
     float pmr = in ? pn[(size_t)t.ya * P + c] : 0.0f;
     float c2r = in ? c2f[(size_t)t.ya * P + c] : 0.0f;
+    const size_t xrow0 = (size_t)(band * NY) * dn + bcol;           // phx/chx of row y at xrow0 + y*dn
+    float phxr = xpml ? phx_o[xrow0 + (size_t)t.ya * dn] : 0.0f;
+    float chxr = xpml ? chx[xrow0 + (size_t)t.ya * dn] : 0.0f;
+    // phi_y window: band rows rc-3 .. rc+3 around the clamped band row rc
+    float q[7];
+    int rc = 0;
+    float chyr = 0.0f;
+    const size_t yrow0 = (size_t)(yband * dn) * P + c;              // phy/chy of band row r at yrow0 + r*P
+#pragma unroll
+    for (int r = 0; r < 7; ++r) q[r] = 0.0f;
+    if (ytask) {
+        rc = clampi(t.ya - yb0, ACC, dn - 1 - ACC);
+        if (in) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) q[r] = phy_o[yrow0 + (size_t)(rc - 3 + r) * P];
+            chyr = chy[yrow0 + (size_t)(t.ya - yb0) * P];
+        }
+    }
 
     int buf = 0;
     for (int y = t.ya; y < t.yb; ++y) {
+        // request the operands of row y+1
         const int y1 = y + 1;
         const bool more = y1 < t.yb;
         const int yc1 = clampi(y1, ACC, NY - 1 - ACC);
         const bool shift = more && (yc1 != yc);
-        float npm = 0.0f, nc2 = 0.0f, npw = 0.0f;
+        const int rc1 = clampi(y1 - yb0, ACC, dn - 1 - ACC);
+        const bool qshift = ytask && more && (rc1 != rc);
+        float npm = 0.0f, nc2 = 0.0f, npw = 0.0f, nphx = 0.0f, nchx = 0.0f, nq = 0.0f, nchy = 0.0f;
         if (more && in) {
             npm = pn[(size_t)y1 * P + c];
             nc2 = c2f[(size_t)y1 * P + c];
+            if (dxv > 0) {
+                nphx = phx_o[xrow0 + (size_t)y1 * dn];
+                nchx = chx[xrow0 + (size_t)y1 * dn];
+            }
+            if (ytask) {
+                nchy = chy[yrow0 + (size_t)(y1 - yb0) * P];
+                if (qshift) nq = phy_o[yrow0 + (size_t)(rc1 + 3) * P];
+            }
         }
         if (shift && ld_ok) npw = pc[(size_t)(yc1 + 3) * P + c];
 
@@ -194,7 +246,8 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
         else if (wr == 4) prow = w[4];
         else if (wr == 5) prow = w[5];
         else if (wr == 6) prow = w[6];
-        lds_p[buf][lane] = prow;
+        L.p[buf][lane] = prow;
+        L.phx[buf][lane] = phxr;
 
         const int ry = y - t.ya;
         const bool has_inj = (imask >> ry) & 1ull;
@@ -204,59 +257,56 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
             ie = prm.rows[ioff + ry + 1];
             for (int e = ib + lane; e < ie; e += 64) {
                 const int2 en = prm.ents[e];
-                lds_src[en.x - x0] = itr_t[en.y];
+                L.src[en.x - x0] = itr_t[en.y];
             }
         }
         __syncthreads();
 
-        const bool ytop = y < dn;
-        const bool ybot = y >= NY - dn;
-        const int dyv = ytop ? dn - y : (ybot ? y - (NY - dn) + 1 : 0);
-        const int yband = ytop ? 0 : 1;
-        const int rb = ytop ? y : y - (NY - dn);
-        const int rbc = clampi(rb, ACC, dn - 1 - ACC);
+        const int dyv = ytask ? (yband == 0 ? dn - y : y - (NY - dn) + 1) : 0;
+        const int rb = y - yb0;                  // band row (y-band tasks)
         const bool model_row = y >= dn && y < dn + prm.ny;
 
         float pp = 0.0f;
         if (in) {
             const int li = clampi(c, ACC, NX - 1 - ACC) - x0;
-            const float* L = &lds_p[buf][li];
-            const float d2x = dd2(L[-3], L[-2], L[-1], L[0], L[1], L[2], L[3]);
+            const float* Lp = &L.p[buf][li];
+            const float d2x = dd2(Lp[-3], Lp[-2], Lp[-1], Lp[0], Lp[1], Lp[2], Lp[3]);
             const float d2y = dd2(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
             const float base = (-pmr) + 2.0f * prow;
             const float lap = d2x + d2y;
             if (dxv == 0 && dyv == 0) {
-                float q = lap;
-                if (has_inj) q = q + lds_src[c - x0];
-                pp = base + q * c2r;
+                float qq = lap;
+                if (has_inj) qq = qq + L.src[c - x0];
+                pp = base + qq * c2r;
             } else {
                 const int m = dxv > dyv ? dxv : dyv;
                 const float A = prm.fA[m], B = prm.fB[m];
                 float propx = 0.0f, propy = 0.0f;
                 if (dxv > 0) {
-                    const float d1x = dd1(L[-3], L[-2], L[-1], L[1], L[2], L[3]);
-                    const int band = c < dn ? 0 : 1;
-                    const int b = band == 0 ? c : c - (NX - dn);
-                    const int bc = clampi(b, ACC, dn - 1 - ACC);
-                    const size_t row = (size_t)(band * NY + y) * dn;
-                    const float* ph = phx_o + row;
-                    const float dph = dd1(ph[bc - 3], ph[bc - 2], ph[bc - 1], ph[bc + 1], ph[bc + 2], ph[bc + 3]);
-                    const float phn = A * ph[b] + B * d1x;
-                    const float chn = A * chx[row + b] + B * (d2x + dph);
-                    phx_n[row + b] = phn;
-                    chx[row + b] = chn;
+                    const float d1x = dd1(Lp[-3], Lp[-2], Lp[-1], Lp[1], Lp[2], Lp[3]);
+                    const float* Lx = &L.phx[buf][lb];
+                    const float dph = dd1(Lx[-3], Lx[-2], Lx[-1], Lx[1], Lx[2], Lx[3]);
+                    const float phn = A * phxr + B * d1x;
+                    const float chn = A * chxr + B * (d2x + dph);
+                    phx_n[xrow0 + (size_t)y * dn] = phn;
+                    chx[xrow0 + (size_t)y * dn] = chn;
                     propx = dph + chn;
                 }
                 if (dyv > 0) {
                     const float d1y = dd1(w[0], w[1], w[2], w[4], w[5], w[6]);
-                    const float* ph = phy_o + (size_t)(yband * dn) * P + c;
-                    const float dph = dd1(ph[(size_t)(rbc - 3) * P], ph[(size_t)(rbc - 2) * P], ph[(size_t)(rbc - 1) * P],
-                                          ph[(size_t)(rbc + 1) * P], ph[(size_t)(rbc + 2) * P], ph[(size_t)(rbc + 3) * P]);
-                    const size_t idx = (size_t)(yband * dn + rb) * P + c;
-                    const float phn = A * phy_o[idx] + B * d1y;
-                    const float chn = A * chy[idx] + B * (d2y + dph);
-                    phy_n[idx] = phn;
-                    chy[idx] = chn;
+                    const int qr = rb - rc + 3;
+                    float qown = q[3];
+                    if (qr == 0) qown = q[0];
+                    else if (qr == 1) qown = q[1];
+                    else if (qr == 2) qown = q[2];
+                    else if (qr == 4) qown = q[4];
+                    else if (qr == 5) qown = q[5];
+                    else if (qr == 6) qown = q[6];
+                    const float dph = dd1(q[0], q[1], q[2], q[4], q[5], q[6]);
+                    const float phn = A * qown + B * d1y;
+                    const float chn = A * chyr + B * (d2y + dph);
+                    phy_n[yrow0 + (size_t)rb * P] = phn;
+                    chy[yrow0 + (size_t)rb * P] = chn;
                     propy = dph + chn;
                 }
                 if (dxv > 0 && dyv > 0) pp = (base + c2r * (lap + propx)) + c2r * propy;
@@ -279,28 +329,37 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
         }
 
         if ((rmask >> ry) & 1ull) {
-            lds_out[lane] = pp;
+            L.out[lane] = pp;
             __syncthreads();
             const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
             for (int e = rbeg + lane; e < rend; e += 64) {
                 const int2 en = prm.ents[e];
-                rec_t[en.y] = lds_out[en.x - x0];
+                rec_t[en.y] = L.out[en.x - x0];
             }
             __syncthreads();
         }
         if (has_inj) {
             __syncthreads();
-            for (int e = ib + lane; e < ie; e += 64) lds_src[prm.ents[e].x - x0] = 0.0f;
+            for (int e = ib + lane; e < ie; e += 64) L.src[prm.ents[e].x - x0] = 0.0f;
             __syncthreads();
         }
 
         pmr = npm;
         c2r = nc2;
+        phxr = nphx;
+        chxr = nchx;
+        chyr = nchy;
         if (shift) {
 #pragma unroll
             for (int r = 0; r < 6; ++r) w[r] = w[r + 1];
             w[6] = npw;
             yc = yc1;
+        }
+        if (qshift) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) q[r] = q[r + 1];
+            q[6] = nq;
+            rc = rc1;
         }
         buf ^= 1;
     }
@@ -310,13 +369,9 @@ __global__ __launch_bounds__(64) void frame_kernel(const StepParams prm) {
 // Interior kernel: all cells of the task are model cells at least acc rows
 // and columns away from the frame, so no clamping and no CPML terms apply.
 // Lane l holds columns x0+4l .. x0+4l+3; lanes 0 and 63 only load halo.
-__global__ __launch_bounds__(64) void interior_kernel(const StepParams prm) {
-    __shared__ float4 lds_src[64];
-    __shared__ float4 lds_out[64];
-
-    const int task = task_index();
-    if (task >= prm.ntasks) return;
-    const Task t = prm.tasks[task];
+__device__ __forceinline__ void interior_task(const StepParams& prm, const Task& t, InteriorLds& L) {
+    float4* lds_src = L.src;
+    float4* lds_out = L.out;
     const ShotDev& s = prm.shots[t.shot];
 
     const int lane = threadIdx.x;
@@ -499,19 +554,21 @@ __global__ __launch_bounds__(64) void interior_kernel(const StepParams prm) {
     }
 }
 
-}  // namespace
-
-hipError_t launch_frame(const StepParams& prm, hipStream_t stream) {
-    const int nb = ((prm.ntasks + 7) / 8) * 8;
-    if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(frame_kernel, dim3(nb), dim3(64), 0, stream, prm);
-    return hipGetLastError();
+// One launch per time step over every shot: frame and interior tasks share
+// the grid, grouped per XCD with the frame tasks first.
+__global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
+    __shared__ StepLds lds;
+    const Task t = prm.tasks[task_index()];
+    if (t.kind == TASK_FRAME) frame_task(prm, t, lds.f);
+    else if (t.kind == TASK_INTERIOR) interior_task(prm, t, lds.i);
 }
 
-hipError_t launch_interior(const StepParams& prm, hipStream_t stream) {
-    const int nb = ((prm.ntasks + 7) / 8) * 8;
-    if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(interior_kernel, dim3(nb), dim3(64), 0, stream, prm);
+}  // namespace
+
+hipError_t launch_step(const StepParams& prm, hipStream_t stream) {
+    if (prm.ntasks == 0) return hipSuccess;
+    if (prm.ntasks % 8 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(step_kernel, dim3(prm.ntasks), dim3(64), 0, stream, prm);
     return hipGetLastError();
 }
 
