@@ -131,7 +131,7 @@ def cpu_baseline(acq, nproc, nt_sample):
         "unit": "Gcell-updates/s",
         "cores": nproc,
         "kind": kind,
-        "sample": (f"{nproc} config-2 shot windows (one per process), forward with snapshots + adjoint with "
+        "sample": (f"{nproc} marine-model shot windows (one per process), forward with snapshots + adjoint with "
                    f"imaging, {nt_sample} time samples each; {cells:.3e} cell-updates in {wall:.1f} s"),
         "seconds_per_gradient_est": full_cells / (rate * 1e9),
     }
@@ -144,7 +144,7 @@ def main():
               f"multi-GPU path is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
     from fwi_amd import synth
-    acq = synth.config2(seed=0, nshots=a.nshots, nt=a.nt)
+    acq = synth.marine_64shot_2000x400(seed=0, nshots=a.nshots, nt=a.nt)
     cpu = None
     if not a.no_cpu:
         nproc = a.cpu_procs or min(16, os.cpu_count() or 1)

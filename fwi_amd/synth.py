@@ -66,7 +66,7 @@ class Acquisition:
         return sum((s["w"] + 2 * dn) * (self.ny + 2 * dn) for s in self.shots) * (self.nt - 1) * sweeps
 
 
-def config1(free_surface: int = 0) -> Acquisition:
+def single_shot_400x150(free_surface: int = 0) -> Acquisition:
     """Single-shot parity case: nx=400, nz=150, h=25 m, dt=2 ms, nt=2000.
 
     Water (1500 m/s) down to 2 km, then a linear 1800 -> 4500 m/s gradient to
@@ -92,13 +92,13 @@ def config1(free_surface: int = 0) -> Acquisition:
     sy = grid_index(10.0, h)
     shot = dict(x0=0, w=nx, src_x=[grid_index(2500.0, h)], src_y=[sy],
                 rec_x=[grid_index(1250.0 + 25.0 * i, h) for i in range(300)], rec_y=[sy] * 300, f0=8.0)
-    acq = Acquisition("config1", ny, nx, h, dt, nt, vt.astype(np.float32), vi.astype(np.float32), wr,
+    acq = Acquisition("single_shot_400x150", ny, nx, h, dt, nt, vt.astype(np.float32), vi.astype(np.float32), wr,
                       [shot], [ricker(8.0, nt, dt)[:, None]], free_surface=free_surface)
     acq.notes = {"anomaly_centre_m": (float(xc), float(zc)), "receiver_start_m": 1250.0}
     return acq
 
 
-def config2(seed: int = 0, nshots: int = 64, nt: int = 5000) -> Acquisition:
+def marine_64shot_2000x400(seed: int = 0, nshots: int = 64, nt: int = 5000) -> Acquisition:
     """Batched-shot workload: nx=2000, nz=400, h=25 m, dt=2 ms, nt=5000, 64 shots.
 
     Seed-0 layered marine model: water (1500 m/s) over a smooth random
@@ -150,7 +150,7 @@ def config2(seed: int = 0, nshots: int = 64, nt: int = 5000) -> Acquisition:
         ry = wr[x0:x0 + w]
         shots.append(dict(x0=x0, w=w, src_x=[sx], src_y=[sy], rec_x=rx.tolist(), rec_y=ry.tolist(), f0=f0))
         wavelets.append(ricker(f0, nt, dt)[:, None])
-    acq = Acquisition("config2", ny, nx, h, dt, nt, vtrue, vinit, wr, shots, wavelets, free_surface=0)
+    acq = Acquisition("marine_64shot_2000x400", ny, nx, h, dt, nt, vtrue, vinit, wr, shots, wavelets, free_surface=0)
     acq.notes = {"seed": seed, "vmax": float(vtrue.max()), "courant": float(vtrue.max()) * dt / h}
     return acq
 

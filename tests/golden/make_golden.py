@@ -6,7 +6,7 @@ Fixtures (numpy .npz, no pickled objects):
   tiny_fs{0,1}_r{4,8}.npz  40x60 model, nt=200, 2 sources, 7 receivers:
                            forward seismograms, all snapshot slots (REAL(4)
                            only), misfit and adjoint source, imaging kernel
-  config1_fs0_r4.npz       config-1 single-shot gradient path (150x400,
+  single_shot_400x150_fs0_r4.npz       single-shot gradient path (150x400,
                            nt=2000): imaging kernel, decimated seismograms,
                            misfit, SHA-256 of the full seismogram and snapshots
   pml_d{20,50}_r{4,8}.npz  CPML profiles and corner matrices (pml_coeffs_)
@@ -92,16 +92,16 @@ def main():
             np.savez_compressed(os.path.join(HERE, f"tiny_{v}.npz"), **out)
             print("wrote", v, "misfit", mis)
 
-    acq = synth.config1(free_surface=0)
+    acq = synth.single_shot_400x150(free_surface=0)
     sh = acq.shots[0]
     f32 = np.float32
     obs, F, mis, adj, A = gradient_path("fs0_r4", acq.vp_true, acq.vp_init, acq.wavelets[0], sh["src_x"],
                                         sh["src_y"], sh["rec_x"], sh["rec_y"], f32(acq.h), f32(acq.dt), acq.nt,
                                         acq.store_snap)
-    np.savez_compressed(os.path.join(HERE, "config1_fs0_r4.npz"), kernel=A.kernel, rec_dec=F.rec[::10, ::10],
+    np.savez_compressed(os.path.join(HERE, "single_shot_400x150_fs0_r4.npz"), kernel=A.kernel, rec_dec=F.rec[::10, ::10],
                         obs_dec=obs[::10, ::10], misfit=mis, adj_dec=adj[::10, ::10],
                         rec_sha=sha(F.rec), obs_sha=sha(obs), adj_sha=sha(adj), fields_sha=sha(F.fields))
-    print("wrote config1 misfit", mis)
+    print("wrote single-shot 400x150 misfit", mis)
 
     for dpml in (20, 50):
         for r in ("r4", "r8"):

@@ -46,10 +46,10 @@ def test_tiny_fixture(fs, prec):
 
 
 @pytest.mark.slow
-def test_config1_fixture():
-    """Config-1 single-shot gradient path (150x400, nt=2000, CPML all round)."""
-    g = _load("config1_fs0_r4.npz")
-    acq = synth.config1(free_surface=0)
+def test_single_shot_400x150_fixture():
+    """Single-shot gradient path (150x400, nt=2000, CPML all round)."""
+    g = _load("single_shot_400x150_fs0_r4.npz")
+    acq = synth.single_shot_400x150(free_surface=0)
     sh = acq.shots[0]
     f32 = np.float32
     args = (sh["src_x"], sh["src_y"], sh["rec_x"], sh["rec_y"])
