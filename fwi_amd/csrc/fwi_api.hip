@@ -22,6 +22,8 @@
 
 namespace {
 
+using fwi::ACC;
+
 thread_local std::string g_err;
 
 int fail(const std::string& m) {
@@ -119,6 +121,7 @@ struct fwi_ctx {
     fwi::ShotDev* d_desc = nullptr;
     fwi::Task* d_tasks = nullptr;         // frame and interior tasks, grouped per XCD
     int ntasks = 0;
+
     int* d_rows = nullptr;
     int2* d_ents = nullptr;
     float* d_c2s = nullptr;               // four shifted copies of the model c2 (interior loads)
@@ -217,7 +220,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     const int dn = c->dn, NY = c->NY, ny = cf.ny;
     c->shots.assign(nshots, Shot());
     c->nrec_total = c->nsrc_total = 0;
-    std::vector<std::vector<fwi::Task>> ftasks(nshots), itasks(nshots);
+    std::vector<std::vector<fwi::Task>> ftasks(nshots), itasks(nshots), ytasks(nshots);
     std::vector<long long> work(nshots, 0);
     std::vector<int> rows;
     std::vector<int2> ents;
@@ -273,22 +276,33 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         for (auto& yr : row_range(dn, NY - dn, R))
             for (auto& xr : tile_range(XL, XR, fwi::interior_tile_cols(), -1))
                 add_task(itasks[i], IT, xr.first, xr.second, yr.first, yr.second);
-        // frame: the left band [0, XL) and right band [XR, NX) each form one
-        // tile; over the model rows they are the whole task, in the bottom
-        // (and top) band rows the columns between them are tiled too
+        // frame, general path: the left band [0, XL) and right band [XR, NX)
+        // each form one tile (with the corners); in the top and bottom bands
+        // the columns between them are tiled for the three rows at each edge
+        // of the band, where derivatives are clamped.  The remaining band
+        // rows between the x bands take the y-band path.
         const int FW = fwi::frame_tile_cols();
-        std::vector<std::pair<int, int>> band_cols = {{0, XL}};
-        for (auto& xr : tile_range(XL, XR, FW, -1)) band_cols.push_back(xr);
-        band_cols.push_back({XR, s.NX});
+        const int YT = fwi::TASK_YBAND;
+        auto mid_cols = tile_range(XL, XR, FW, -1);
         for (auto& yr : row_range(dn, NY - dn, R)) {
             add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
             add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
         }
-        for (auto& yr : row_range(NY - dn, NY, R))
-            for (auto& xr : band_cols) add_task(ftasks[i], FT, xr.first, xr.second, yr.first, yr.second);
-        if (!cf.free_surface)
-            for (auto& yr : row_range(0, dn, R))
-                for (auto& xr : band_cols) add_task(ftasks[i], FT, xr.first, xr.second, yr.first, yr.second);
+        auto band = [&](int b0) {
+            for (auto& yr : row_range(b0, b0 + dn, R)) {
+                add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
+                add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
+            }
+            for (auto& xr : mid_cols) {
+                add_task(ftasks[i], FT, xr.first, xr.second, b0, b0 + ACC);
+                add_task(ftasks[i], FT, xr.first, xr.second, b0 + dn - ACC, b0 + dn);
+            }
+            for (auto& yr : row_range(b0 + ACC, b0 + dn - ACC, R))
+                for (auto& xr : tile_range(XL, XR, fwi::yband_tile_cols(), -1))
+                    add_task(ytasks[i], YT, xr.first, xr.second, yr.first, yr.second);
+        };
+        band(NY - dn);
+        if (!cf.free_surface) band(0);
         work[i] = (long long)(NY - (cf.free_surface ? dn : 0)) * s.NX;
 
         s.roff = c->nrec_total;
@@ -342,12 +356,32 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         groups[g].push_back(i);
         load[g] += work[i];
     }
+    // Within a group the task kinds are interleaved in proportion (a
+    // latency-bound frame task next to bandwidth-bound interior tasks keeps
+    // every CU busy on both).
     std::vector<std::vector<fwi::Task>> gt(8);
     size_t per = 0;
     for (int g = 0; g < 8; ++g) {
         std::sort(groups[g].begin(), groups[g].end());
-        for (int i : groups[g]) gt[g].insert(gt[g].end(), ftasks[i].begin(), ftasks[i].end());
-        for (int i : groups[g]) gt[g].insert(gt[g].end(), itasks[i].begin(), itasks[i].end());
+        std::vector<fwi::Task> kinds[3];
+        for (int i : groups[g]) {
+            kinds[0].insert(kinds[0].end(), ftasks[i].begin(), ftasks[i].end());
+            kinds[1].insert(kinds[1].end(), ytasks[i].begin(), ytasks[i].end());
+            kinds[2].insert(kinds[2].end(), itasks[i].begin(), itasks[i].end());
+        }
+        const size_t total = kinds[0].size() + kinds[1].size() + kinds[2].size();
+        size_t taken[3] = {0, 0, 0};
+        for (size_t n = 0; n < total; ++n) {
+            // issue the kind that is furthest behind its share
+            int best = -1;
+            double lag = -1e300;
+            for (int q = 0; q < 3; ++q) {
+                if (taken[q] >= kinds[q].size()) continue;
+                const double d = (double)(n + 1) * kinds[q].size() / total - taken[q];
+                if (d > lag) { lag = d; best = q; }
+            }
+            gt[g].push_back(kinds[best][taken[best]++]);
+        }
         per = std::max(per, gt[g].size());
     }
     std::vector<fwi::Task> tasks(8 * per);
@@ -486,6 +520,7 @@ int run(fwi_ctx* c, int mode, bool store) {
         HIPCHK(hipEventCreate(&ev1));
         HIPCHK(hipEventRecord(ev0, c->stream));
     }
+
     for (int k = 2; k <= cf.nt; ++k) {
         prm.k = k;
         prm.snap_op = 0;

@@ -56,7 +56,7 @@ struct Task {
     unsigned long long rec_mask;
 };
 
-enum TaskKind : int { TASK_IDLE = -1, TASK_FRAME = 0, TASK_INTERIOR = 1 };
+enum TaskKind : int { TASK_IDLE = -1, TASK_FRAME = 0, TASK_INTERIOR = 1, TASK_YBAND = 2 };
 
 // Per-launch parameters of the time-step kernels.
 struct StepParams {
@@ -86,6 +86,7 @@ constexpr int frame_halo_lanes() { return (ACC + FRAME_VX - 1) / FRAME_VX; }
 constexpr int frame_tile_cols() { return (64 - 2 * frame_halo_lanes()) * FRAME_VX; }
 constexpr int INTERIOR_VX = 4;                                      // interior kernel: 4 columns per lane
 constexpr int interior_tile_cols() { return (64 - 2) * INTERIOR_VX; }   // lanes 0 and 63 only load halo
+constexpr int yband_tile_cols() { return (64 - 4) * 2; }                  // 2 columns per lane, 2 halo lanes a side
 
 // One launch advances every shot by one step; ntasks must be a multiple of
 // 8 (tasks grouped per XCD, padded with TASK_IDLE).

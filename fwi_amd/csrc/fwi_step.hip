@@ -554,13 +554,147 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
     }
 }
 
-// One launch per time step over every shot: frame and interior tasks share
-// the grid, grouped per XCD with the frame tasks first.
+// ---------------------------------------------------------------------------
+// Y-band task: rows 3..dn-4 of the top or bottom CPML band (counted from the
+// band's first row), columns strictly between the x bands.  There the update
+// has only the y-CPML term, no derivative is clamped (neither p's nor the
+// band-local one of phi_y) and the CPML factor depends on the row only.
+// Two columns per lane (lanes 0,1 and 62,63 only load halo), x stencil
+// through DPP, p and phi_y in 8-slot register windows rotated by unrolling.
+__device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
+__device__ __forceinline__ void st2(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
+
+__device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t, InteriorLds& L) {
+    float2* lds_out = reinterpret_cast<float2*>(L.out);
+    const ShotDev& s = prm.shots[t.shot];
+
+    const int lane = threadIdx.x;
+    const int P = s.P, dn = prm.dn, NY = prm.NY;
+    const int x0 = t.xs - 4;
+    const int xl = x0 + 2 * lane;
+    const bool ld = xl <= t.xe + 2;
+    const bool cl = lane >= 2 && lane < 62 && xl < t.xe;
+    const bool full = cl && xl + 2 <= t.xe;
+    const int cur = prm.k & 1;
+    const float* __restrict__ pc = s.p[cur];
+    float* __restrict__ pn = s.p[cur ^ 1];
+    const int yband = t.ya < dn ? 0 : 1;
+    const int yb0 = yband == 0 ? 0 : NY - dn;
+    const size_t brow0 = (size_t)(yband * dn) * P + xl;            // band row r at brow0 + r*P
+    const float* __restrict__ qo = s.phy[cur] + brow0;
+    float* __restrict__ qn = s.phy[cur ^ 1] + brow0;
+    float* __restrict__ chy = s.chy + brow0;
+    const float* lanes_p = pc + xl;
+    float* lanes_pn = pn + xl;
+    const float* lanes_c2 = s.c2 + xl;
+    const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
+    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
+
+    const float2 z2 = make_float2(0.0f, 0.0f);
+    float2 S[8], Q[8];
+    const int rb_a = t.ya - yb0;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        S[r] = ld ? ld2(lanes_p + (size_t)(t.ya - 3 + r) * P) : z2;
+        Q[r] = cl ? ld2(qo + (size_t)(rb_a - 3 + r) * P) : z2;
+    }
+    S[7] = z2;
+    Q[7] = z2;
+    float2 PM[2], C2[2], CH[2];
+    PM[0] = cl ? ld2(lanes_pn + (size_t)t.ya * P) : z2;
+    C2[0] = cl ? ld2(lanes_c2 + (size_t)t.ya * P) : z2;
+    CH[0] = cl ? ld2(chy + (size_t)rb_a * P) : z2;
+    PM[1] = C2[1] = CH[1] = z2;
+
+    auto row = [&](int y0, auto Jc) -> bool {
+        constexpr int j = decltype(Jc)::value;
+        const int y = y0 + j;
+        if (y >= t.yb) return false;
+        const int rb = y - yb0;
+        if (y + 1 < t.yb) {
+            if (ld) S[(j + 7) & 7] = ld2(lanes_p + (size_t)(y + 4) * P);
+            if (cl) {
+                Q[(j + 7) & 7] = ld2(qo + (size_t)(rb + 4) * P);
+                PM[(j + 1) & 1] = ld2(lanes_pn + (size_t)(y + 1) * P);
+                C2[(j + 1) & 1] = ld2(lanes_c2 + (size_t)(y + 1) * P);
+                CH[(j + 1) & 1] = ld2(chy + (size_t)(rb + 1) * P);
+            }
+        }
+        const float2 m3 = S[j & 7], m2 = S[(j + 1) & 7], m1 = S[(j + 2) & 7], c = S[(j + 3) & 7];
+        const float2 p1 = S[(j + 4) & 7], p2 = S[(j + 5) & 7], p3 = S[(j + 6) & 7];
+        const float2 q3 = Q[j & 7], q2 = Q[(j + 1) & 7], q1 = Q[(j + 2) & 7], qc = Q[(j + 3) & 7];
+        const float2 r1 = Q[(j + 4) & 7], r2 = Q[(j + 5) & 7], r3 = Q[(j + 6) & 7];
+        const float2 pm = PM[j & 1], cc = C2[j & 1], ch = CH[j & 1];
+
+        // columns x-3 .. x+4 around this lane's pair (x, x+1)
+        const float lm1 = from_left(c.y), lm2 = from_left(c.x), lm3 = from_left(lm1);
+        const float rp2 = from_right(c.x), rp3 = from_right(c.y), rp4 = from_right(rp2);
+
+        const int dyv = yband == 0 ? dn - y : y - (NY - dn) + 1;
+        const float A = prm.fA[dyv], B = prm.fB[dyv];
+        float2 pp, phn, chn;
+#define FWI_YBAND_CELL(F, D2X)                                                                  \
+        {                                                                                       \
+            const float d2x = D2X;                                                              \
+            const float d2y = dd2(m3.F, m2.F, m1.F, c.F, p1.F, p2.F, p3.F);                     \
+            const float d1y = dd1(m3.F, m2.F, m1.F, p1.F, p2.F, p3.F);                          \
+            const float dph = dd1(q3.F, q2.F, q1.F, r1.F, r2.F, r3.F);                          \
+            phn.F = A * qc.F + B * d1y;                                                         \
+            chn.F = A * ch.F + B * (d2y + dph);                                                 \
+            const float propy = dph + chn.F;                                                    \
+            pp.F = ((-pm.F) + 2.0f * c.F) + cc.F * ((d2x + d2y) + propy);                       \
+        }
+        FWI_YBAND_CELL(x, dd2(lm3, lm2, lm1, c.x, c.y, rp2, rp3))
+        FWI_YBAND_CELL(y, dd2(lm2, lm1, c.x, c.y, rp2, rp3, rp4))
+#undef FWI_YBAND_CELL
+
+        float* dst = lanes_pn + (size_t)y * P;
+        float* dq = qn + (size_t)rb * P;
+        float* dc = chy + (size_t)rb * P;
+        if (full) {
+            st2(dst, pp);
+            st2(dq, phn);
+            st2(dc, chn);
+        } else if (cl) {
+            dst[0] = pp.x;
+            dq[0] = phn.x;
+            dc[0] = chn.x;
+        }
+
+        const int ry = y - t.ya;
+        if ((rmask >> ry) & 1ull) {
+            lds_out[lane] = pp;
+            __syncthreads();
+            const float* lo = reinterpret_cast<const float*>(lds_out);
+            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            for (int e = rbeg + lane; e < rend; e += 64) {
+                const int2 en = prm.ents[e];
+                rec_t[en.y] = lo[en.x - x0];
+            }
+            __syncthreads();
+        }
+        return true;
+    };
+    for (int y0 = t.ya; y0 < t.yb; y0 += 8) {
+        if (!row(y0, std::integral_constant<int, 0>{})) break;
+        if (!row(y0, std::integral_constant<int, 1>{})) break;
+        if (!row(y0, std::integral_constant<int, 2>{})) break;
+        if (!row(y0, std::integral_constant<int, 3>{})) break;
+        if (!row(y0, std::integral_constant<int, 4>{})) break;
+        if (!row(y0, std::integral_constant<int, 5>{})) break;
+        if (!row(y0, std::integral_constant<int, 6>{})) break;
+        if (!row(y0, std::integral_constant<int, 7>{})) break;
+    }
+}
+
+// One launch per time step over every shot: frame, y-band and interior tasks
+// share the grid, grouped per XCD and interleaved in proportion.
 __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
     __shared__ StepLds lds;
     const Task t = prm.tasks[task_index()];
     if (t.kind == TASK_FRAME) frame_task(prm, t, lds.f);
     else if (t.kind == TASK_INTERIOR) interior_task(prm, t, lds.i);
+    else if (t.kind == TASK_YBAND) yband_task(prm, t, lds.i);
 }
 
 }  // namespace
