@@ -216,7 +216,7 @@ def main():
             "nshots": len(acq.shots),
             "nt": acq.nt,
             "snapshots": a.snap,
-            "rows_per_task": a.rows or 32,
+            "rows_per_task": a.rows or 8,
         },
         "seconds_per_gradient": wall / a.steps,
         "misfit": misfit,

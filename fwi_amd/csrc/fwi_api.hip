@@ -605,7 +605,7 @@ int ctx_init(fwi_ctx* c, const fwi_config* cfg, int nt_store) {
     c->dn = cfg->dpml + 2 * fwi::ACC;
     c->NY = cfg->ny + 2 * c->dn;
     c->nt_store = nt_store;
-    c->rows = cfg->rows_per_task > 0 ? std::min(cfg->rows_per_task, fwi::MAX_TASK_ROWS) : 32;
+    c->rows = cfg->rows_per_task > 0 ? std::min(cfg->rows_per_task, fwi::MAX_TASK_ROWS) : 8;
     cpml_table(cfg->dpml, cfg->dt, cfg->d, c->dn, c->fA, c->fB);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     return 0;

@@ -94,7 +94,7 @@ typedef struct fwi_config {
     int dpml;              /* CPML thickness (reference: 20) */
     int free_surface;      /* as in fwi_solver_opts */
     int snap_kind;         /* FWI_SNAP_F32 / _BF16 / _F16 */
-    int rows_per_task;     /* rows per kernel work unit, 1..64 (0 = default 32) */
+    int rows_per_task;     /* rows per kernel work unit, 1..64 (0 = default 8) */
 } fwi_config;
 
 /* One shot: an aperture window of the model plus its acquisition, positions
