@@ -163,7 +163,9 @@ int validate_config(const fwi_config& c) {
     if (c.ny < 1 || c.nx < 1) return fail("model dimensions must be positive");
     if (c.nt < 2) return fail("nt must be at least 2");
     if (c.store_snap < 1) return fail("store_snap must be positive");
-    if (c.dpml < 1 || c.dpml + 2 * fwi::ACC > fwi::MAXDN) return fail("dpml out of range");
+    // the CPML profile divides by (dpml - 1) * d (solver_PML_coeffs.f90:19-24)
+    if (c.dpml < 2) return fail("dpml must be >= 2 (the CPML profile of dpml = 1 is 0/0)");
+    if (c.dpml + 2 * fwi::ACC > fwi::MAXDN) return fail("dpml out of range");
     if (!(c.d > 0.0f) || !(c.dt > 0.0f)) return fail("d and dt must be positive");
     if (c.snap_kind != fwi::SNAP_F32 && c.snap_kind != fwi::SNAP_BF16 && c.snap_kind != fwi::SNAP_F16)
         return fail("unknown snapshot kind");
@@ -198,12 +200,19 @@ std::vector<std::pair<int, int>> row_range(int a, int b, int maxr) {
 // Per-task row tables of one cell set: rows[off + r] .. rows[off + r + 1]
 // index the task's entries on row ya + r; bit r of the mask is set when the
 // row has entries.
-void attach_cells(fwi::Task& t, const std::vector<Cell>& cells, std::vector<int>& rows, std::vector<int2>& ents,
-                  int& off, unsigned long long& mask) {
+// A pair task owns the columns outside [xs, xe), every other kind the
+// columns inside.
+bool task_owns_column(const fwi::Task& t, int x, int NX) {
+    if (t.kind == fwi::TASK_XPAIR) return (x >= 0 && x < t.xs) || (x >= t.xe && x < NX);
+    return x >= t.xs && x < t.xe;
+}
+
+void attach_cells(fwi::Task& t, int NX, const std::vector<Cell>& cells, std::vector<int>& rows,
+                  std::vector<int2>& ents, int& off, unsigned long long& mask) {
     const int nr = t.yb - t.ya;
     std::vector<std::vector<int2>> per(nr);
     for (const Cell& c : cells)
-        if (c.Y >= t.ya && c.Y < t.yb && c.X >= t.xs && c.X < t.xe) per[c.Y - t.ya].push_back(int2{c.X, c.tr});
+        if (c.Y >= t.ya && c.Y < t.yb && task_owns_column(t, c.X, NX)) per[c.Y - t.ya].push_back(int2{c.X, c.tr});
     off = (int)rows.size();
     mask = 0ull;
     for (int r = 0; r < nr; ++r) {
@@ -265,8 +274,8 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
             fwi::Task t;
             std::memset(&t, 0, sizeof(t));
             t.kind = kind; t.shot = i; t.xs = xs; t.xe = xe; t.ya = ya; t.yb = yb;
-            for (int dir = 0; dir < 2; ++dir) attach_cells(t, inj[dir], rows, ents, t.inj_off[dir], t.inj_mask[dir]);
-            attach_cells(t, rec, rows, ents, t.rec_off, t.rec_mask);
+            for (int dir = 0; dir < 2; ++dir) attach_cells(t, s.NX, inj[dir], rows, ents, t.inj_off[dir], t.inj_mask[dir]);
+            attach_cells(t, s.NX, rec, rows, ents, t.rec_off, t.rec_mask);
             list.push_back(t);
         };
         // interior: model rows, 4-aligned columns [XL, XR) away from the frame
@@ -276,33 +285,32 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         for (auto& yr : row_range(dn, NY - dn, R))
             for (auto& xr : tile_range(XL, XR, fwi::interior_tile_cols(), -1))
                 add_task(itasks[i], IT, xr.first, xr.second, yr.first, yr.second);
-        // frame, general path: the left band [0, XL) and right band [XR, NX)
-        // each form one tile (with the corners); in the top and bottom bands
-        // the columns between them are tiled for the three rows at each edge
-        // of the band, where derivatives are clamped.  The remaining band
-        // rows between the x bands take the y-band path.
-        const int FW = fwi::frame_tile_cols();
-        const int YT = fwi::TASK_YBAND;
-        auto mid_cols = tile_range(XL, XR, FW, -1);
-        for (auto& yr : row_range(dn, NY - dn, R)) {
-            add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
-            add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
-        }
-        auto band = [&](int b0) {
-            for (auto& yr : row_range(b0, b0 + dn, R)) {
-                add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
-                add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
+        // frame.  x bands (with the corners): both bands of a shot in one
+        // pair task where a band and its halo fit in half a wavefront, else
+        // one general task per band; row chunks never straddle a y-band edge.
+        // y bands between the x bands: column tiles over row chunks of the
+        // band (dn >= 8).
+        const bool pair = dn <= fwi::XPAIR_MAX_DN && XL + ACC <= 32 && s.NX >= 64;
+        const int YT = fwi::TASK_YBAND, PT = fwi::TASK_XPAIR;
+        auto xbands = [&](int a, int b) {
+            for (auto& yr : row_range(a, b, R)) {
+                if (pair) {
+                    add_task(ftasks[i], PT, XL, XR, yr.first, yr.second);
+                } else {
+                    add_task(ftasks[i], FT, 0, XL, yr.first, yr.second);
+                    add_task(ftasks[i], FT, XR, s.NX, yr.first, yr.second);
+                }
             }
-            for (auto& xr : mid_cols) {
-                add_task(ftasks[i], FT, xr.first, xr.second, b0, b0 + ACC);
-                add_task(ftasks[i], FT, xr.first, xr.second, b0 + dn - ACC, b0 + dn);
-            }
-            for (auto& yr : row_range(b0 + ACC, b0 + dn - ACC, R))
+        };
+        auto yband = [&](int b0) {
+            xbands(b0, b0 + dn);
+            for (auto& yr : row_range(b0, b0 + dn, R))
                 for (auto& xr : tile_range(XL, XR, fwi::yband_tile_cols(), -1))
                     add_task(ytasks[i], YT, xr.first, xr.second, yr.first, yr.second);
         };
-        band(NY - dn);
-        if (!cf.free_surface) band(0);
+        xbands(dn, NY - dn);
+        yband(NY - dn);
+        if (!cf.free_surface) yband(0);
         work[i] = (long long)(NY - (cf.free_surface ? dn : 0)) * s.NX;
 
         s.roff = c->nrec_total;

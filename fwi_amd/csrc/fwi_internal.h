@@ -56,7 +56,17 @@ struct Task {
     unsigned long long rec_mask;
 };
 
-enum TaskKind : int { TASK_IDLE = -1, TASK_FRAME = 0, TASK_INTERIOR = 1, TASK_YBAND = 2 };
+// TASK_FRAME     general update of columns [xs, xe) (any cell of the frame)
+// TASK_INTERIOR  model cells >= acc away from the frame, 4 columns per lane
+// TASK_YBAND     rows [ya, yb) of the top or bottom band (dn >= 8), columns
+//                [xs, xe) between the x bands, 2 columns per lane
+// TASK_XPAIR     rows [ya, yb) of both x bands of a shot, left band
+//                [0, xs) in lanes 0..31 and right band [xe, NX) in lanes
+//                32..63 (requires xs + acc <= 32, NX - xe + acc <= 32, NX >= 64)
+enum TaskKind : int { TASK_IDLE = -1, TASK_FRAME = 0, TASK_INTERIOR = 1, TASK_YBAND = 2, TASK_XPAIR = 3 };
+
+// Largest frame width dn for which one x band and its halo fit in 32 lanes.
+constexpr int XPAIR_MAX_DN = 32 - ACC - 1;
 
 // Per-launch parameters of the time-step kernels.
 struct StepParams {

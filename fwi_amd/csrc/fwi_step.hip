@@ -42,6 +42,15 @@
 namespace fwi {
 namespace {
 
+// Device memory is addressed through the global address space, so loads and
+// stores compile to global_* instructions.  Generic (flat) accesses also count
+// against the LDS counter, which makes the hardware counters complete out of
+// order for the compiler: every wait then drains all loads in flight,
+// including the operands requested a row ahead.
+#define GMEM __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GMEM T* gmem(T* p) { return (GMEM T*)p; }
+
 // solver_FD_coeffs.f90:14-19 (first derivative) and :43-50,68 (second
 // derivative, centre weight halved), evaluated in float.
 constexpr float C1_1 = 3.0f / 4.0f;
@@ -81,27 +90,39 @@ __device__ __forceinline__ float from_right(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
 }
 
-__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+// Vector accesses go through clang vector types (HIP's float4/float2 are
+// classes whose copy constructors do not take global-space references).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef int i2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld4(const GMEM float* p) {
+    const f4v v = *reinterpret_cast<const GMEM f4v*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4(GMEM float* p, float4 v) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    *reinterpret_cast<GMEM f4v*>(p) = w;
+}
 
-__device__ __forceinline__ void snap_store(void* base, size_t idx, float v, int kind, float inv_scale) {
+__device__ __forceinline__ void snap_store(GMEM void* base, size_t idx, float v, int kind, float inv_scale) {
     if (kind == SNAP_F32) {
-        static_cast<float*>(base)[idx] = v;
+        static_cast<GMEM float*>(base)[idx] = v;
     } else if (kind == SNAP_BF16) {
-        static_cast<__hip_bfloat16*>(base)[idx] = __float2bfloat16(v * inv_scale);
+        static_cast<GMEM unsigned short*>(base)[idx] = __bfloat16_as_ushort(__float2bfloat16(v * inv_scale));
     } else {
-        static_cast<__half*>(base)[idx] = __float2half(v * inv_scale);
+        static_cast<GMEM unsigned short*>(base)[idx] = __half_as_ushort(__float2half(v * inv_scale));
     }
 }
 
-__device__ __forceinline__ float snap_load(const void* base, size_t idx, int kind, float scale) {
-    if (kind == SNAP_F32) return static_cast<const float*>(base)[idx];
-    if (kind == SNAP_BF16) return __bfloat162float(static_cast<const __hip_bfloat16*>(base)[idx]) * scale;
-    return __half2float(static_cast<const __half*>(base)[idx]) * scale;
+__device__ __forceinline__ float snap_load(const GMEM void* base, size_t idx, int kind, float scale) {
+    if (kind == SNAP_F32) return static_cast<const GMEM float*>(base)[idx];
+    const unsigned short b = static_cast<const GMEM unsigned short*>(base)[idx];
+    if (kind == SNAP_BF16) return __bfloat162float(__ushort_as_bfloat16(b)) * scale;
+    return __half2float(__ushort_as_half(b)) * scale;
 }
 
-__device__ __forceinline__ void* slot_base(const ShotDev& s, const StepParams& prm) {
-    return static_cast<char*>(s.snap) + (size_t)prm.slot * s.snap_stride * (prm.snap_kind == SNAP_F32 ? 4 : 2);
+__device__ __forceinline__ GMEM void* slot_base(const GMEM ShotDev& s, const StepParams& prm) {
+    return gmem(static_cast<char*>(s.snap)) + (size_t)prm.slot * s.snap_stride * (prm.snap_kind == SNAP_F32 ? 4 : 2);
 }
 
 // Hardware hands block b to XCD b % 8; give each XCD a contiguous range of
@@ -136,7 +157,10 @@ union StepLds {
 // exchanged through LDS.
 __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t, FrameLds& L) {
     constexpr int H = frame_halo_lanes();
-    const ShotDev& s = prm.shots[t.shot];
+    const GMEM ShotDev& s = gmem(prm.shots)[t.shot];
+    const GMEM int* __restrict__ rowtab = gmem(prm.rows);
+    const GMEM i2v* __restrict__ enttab = reinterpret_cast<const GMEM i2v*>(gmem(prm.ents));
+    GMEM float* __restrict__ grad = gmem(s.grad);
 
     const int lane = threadIdx.x;
     const int NY = prm.NY, dn = prm.dn, NX = s.NX, P = s.P;
@@ -145,22 +169,22 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
     const bool ld_ok = c >= t.xs - ACC && c < t.xe + ACC && c < P;
     const bool in = lane >= H && lane < 64 - H && c >= t.xs && c < t.xe;
     const int cur = prm.k & 1;
-    const float* __restrict__ pc = s.p[cur];     // time k-1
-    float* __restrict__ pn = s.p[cur ^ 1];       // time k-2, overwritten with time k
-    const float* __restrict__ c2f = s.c2;
-    const float* __restrict__ phx_o = s.phx[cur];
-    float* __restrict__ phx_n = s.phx[cur ^ 1];
-    const float* __restrict__ phy_o = s.phy[cur];
-    float* __restrict__ phy_n = s.phy[cur ^ 1];
-    float* __restrict__ chx = s.chx;
-    float* __restrict__ chy = s.chy;
+    const GMEM float* __restrict__ pc = gmem(s.p[cur]);     // time k-1
+    GMEM float* __restrict__ pn = gmem(s.p[cur ^ 1]);       // time k-2, overwritten with time k
+    const GMEM float* __restrict__ c2f = gmem(s.c2);
+    const GMEM float* __restrict__ phx_o = gmem(s.phx[cur]);
+    GMEM float* __restrict__ phx_n = gmem(s.phx[cur ^ 1]);
+    const GMEM float* __restrict__ phy_o = gmem(s.phy[cur]);
+    GMEM float* __restrict__ phy_n = gmem(s.phy[cur ^ 1]);
+    GMEM float* __restrict__ chx = gmem(s.chx);
+    GMEM float* __restrict__ chy = gmem(s.chy);
 
     const int mode = prm.mode;
     const unsigned long long imask = mode ? t.inj_mask[1] : t.inj_mask[0];
     const int ioff = mode ? t.inj_off[1] : t.inj_off[0];
-    const float* __restrict__ itr_t = (mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]);
+    const GMEM float* __restrict__ itr_t = gmem((mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]));
     const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
-    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
+    GMEM float* __restrict__ rec_t = gmem(s.rec_tr + (size_t)(prm.k - 2) * s.nrec);
 
     // x band of this lane
     const int dxv = c < dn ? dn - c : (c >= NX - dn ? c - (NX - dn) + 1 : 0);
@@ -176,11 +200,11 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
     const int yb0 = yband == 0 ? 0 : NY - dn;                       // first row of the band
 
     float snap_scale = 1.0f, snap_inv = 1.0f;
-    void* snap_slot = nullptr;
+    GMEM void* snap_slot = nullptr;
     if (prm.snap_op != 0) {
         snap_slot = slot_base(s, prm);
         if (prm.snap_kind != SNAP_F32 && s.snap_scale) {
-            snap_scale = s.snap_scale[prm.slot];
+            snap_scale = gmem(s.snap_scale)[prm.slot];
             snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
         }
     }
@@ -253,10 +277,10 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
         const bool has_inj = (imask >> ry) & 1ull;
         int ib = 0, ie = 0;
         if (has_inj) {
-            ib = prm.rows[ioff + ry];
-            ie = prm.rows[ioff + ry + 1];
+            ib = rowtab[ioff + ry];
+            ie = rowtab[ioff + ry + 1];
             for (int e = ib + lane; e < ie; e += 64) {
-                const int2 en = prm.ents[e];
+                const i2v en = enttab[e];
                 L.src[en.x - x0] = itr_t[en.y];
             }
         }
@@ -323,7 +347,7 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
-                    s.grad[mi] = s.grad[mi] + k2;
+                    grad[mi] = grad[mi] + k2;
                 }
             }
         }
@@ -331,16 +355,16 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
         if ((rmask >> ry) & 1ull) {
             L.out[lane] = pp;
             __syncthreads();
-            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            const int rbeg = rowtab[t.rec_off + ry], rend = rowtab[t.rec_off + ry + 1];
             for (int e = rbeg + lane; e < rend; e += 64) {
-                const int2 en = prm.ents[e];
+                const i2v en = enttab[e];
                 rec_t[en.y] = L.out[en.x - x0];
             }
             __syncthreads();
         }
         if (has_inj) {
             __syncthreads();
-            for (int e = ib + lane; e < ie; e += 64) L.src[prm.ents[e].x - x0] = 0.0f;
+            for (int e = ib + lane; e < ie; e += 64) L.src[enttab[e].x - x0] = 0.0f;
             __syncthreads();
         }
 
@@ -365,6 +389,252 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
     }
 }
 
+// Value of v held by lane `src` (all 64 lanes active).
+__device__ __forceinline__ float lane_read(float v, int src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+}
+
+// ---------------------------------------------------------------------------
+// X-band pair task: both x bands of a shot over rows [ya, yb), one column per
+// lane.  Lanes 0..31 hold columns 0..31 (the left band [0, XL) and its right
+// halo), lanes 32..63 hold columns NX-32..NX-1 (the right band [XR, NX) and
+// its left halo).  The x stencils of p and of the band-local phi_x run on DPP
+// wave shifts over the row held in registers; the clamped columns (the outer
+// acc columns of the grid for p, the outer acc columns of each band for
+// phi_x) take the derivative of the nearest computed column through a lane
+// read.  Rows march with a 7-row window centred on the clamped row as in the
+// general frame task; a task lies entirely inside or entirely outside a
+// y band.
+__device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t, FrameLds& L) {
+    const GMEM ShotDev& s = gmem(prm.shots)[t.shot];
+    const GMEM int* __restrict__ rowtab = gmem(prm.rows);
+    const GMEM i2v* __restrict__ enttab = reinterpret_cast<const GMEM i2v*>(gmem(prm.ents));
+    GMEM float* __restrict__ grad = gmem(s.grad);
+
+    const int lane = threadIdx.x;
+    const int half = lane >> 5;
+    const int NY = prm.NY, dn = prm.dn, NX = s.NX, P = s.P;
+    const int XL = t.xs, XR = t.xe;
+    const int cb = half ? NX - 32 : 0;           // column of the half's first lane
+    const int c = cb + (lane & 31);
+    const bool in = half ? c >= XR : c < XL;
+    const int cur = prm.k & 1;
+    const GMEM float* __restrict__ pc = gmem(s.p[cur]);
+    GMEM float* __restrict__ pn = gmem(s.p[cur ^ 1]);
+    const GMEM float* __restrict__ c2f = gmem(s.c2);
+    const GMEM float* __restrict__ phx_o = gmem(s.phx[cur]);
+    GMEM float* __restrict__ phx_n = gmem(s.phx[cur ^ 1]);
+    const GMEM float* __restrict__ phy_o = gmem(s.phy[cur]);
+    GMEM float* __restrict__ phy_n = gmem(s.phy[cur ^ 1]);
+    GMEM float* __restrict__ chx = gmem(s.chx);
+    GMEM float* __restrict__ chy = gmem(s.chy);
+
+    const int mode = prm.mode;
+    const unsigned long long imask = mode ? t.inj_mask[1] : t.inj_mask[0];
+    const int ioff = mode ? t.inj_off[1] : t.inj_off[0];
+    const GMEM float* __restrict__ itr_t = gmem((mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]));
+    const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
+    GMEM float* __restrict__ rec_t = gmem(s.rec_tr + (size_t)(prm.k - 2) * s.nrec);
+
+    const int dxv = c < dn ? dn - c : (c >= NX - dn ? c - (NX - dn) + 1 : 0);
+    const bool xpml = in && dxv > 0;
+    const int band = half;
+    const int bcol = half ? c - (NX - dn) : c;                            // column inside the band
+    const int xsrc = lane + (clampi(c, ACC, NX - 1 - ACC) - c);           // lane of the clamped column
+    const int bsrc = lane + (clampi(bcol, ACC, dn - 1 - ACC) - bcol);     // lane of the clamped band column
+    const bool model_col = c >= dn && c < dn + s.W;
+    const bool ytask = t.ya < dn || t.ya >= NY - dn;
+    const int yband = t.ya < dn ? 0 : 1;
+    const int yb0 = yband == 0 ? 0 : NY - dn;
+
+    float snap_scale = 1.0f, snap_inv = 1.0f;
+    GMEM void* snap_slot = nullptr;
+    if (prm.snap_op != 0) {
+        snap_slot = slot_base(s, prm);
+        if (prm.snap_kind != SNAP_F32 && s.snap_scale) {
+            snap_scale = gmem(s.snap_scale)[prm.slot];
+            snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
+        }
+    }
+
+    L.src[lane] = 0.0f;
+
+    float w[7];
+    int yc = clampi(t.ya, ACC, NY - 1 - ACC);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) w[r] = pc[(size_t)(yc - 3 + r) * P + c];
+    float pmr = in ? pn[(size_t)t.ya * P + c] : 0.0f;
+    float c2r = in ? c2f[(size_t)t.ya * P + c] : 0.0f;
+    const size_t xrow0 = (size_t)(band * NY) * dn + bcol;
+    float phxr = xpml ? phx_o[xrow0 + (size_t)t.ya * dn] : 0.0f;
+    float chxr = xpml ? chx[xrow0 + (size_t)t.ya * dn] : 0.0f;
+    float q[7];
+    int rc = 0;
+    float chyr = 0.0f;
+    const size_t yrow0 = (size_t)(yband * dn) * P + c;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) q[r] = 0.0f;
+    if (ytask) {
+        rc = clampi(t.ya - yb0, ACC, dn - 1 - ACC);
+        if (in) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) q[r] = phy_o[yrow0 + (size_t)(rc - 3 + r) * P];
+            chyr = chy[yrow0 + (size_t)(t.ya - yb0) * P];
+        }
+    }
+
+    for (int y = t.ya; y < t.yb; ++y) {
+        const int y1 = y + 1;
+        const bool more = y1 < t.yb;
+        const int yc1 = clampi(y1, ACC, NY - 1 - ACC);
+        const bool shift = more && (yc1 != yc);
+        const int rc1 = clampi(y1 - yb0, ACC, dn - 1 - ACC);
+        const bool qshift = ytask && more && (rc1 != rc);
+        float npm = 0.0f, nc2 = 0.0f, npw = 0.0f, nphx = 0.0f, nchx = 0.0f, nq = 0.0f, nchy = 0.0f;
+        if (more && in) {
+            npm = pn[(size_t)y1 * P + c];
+            nc2 = c2f[(size_t)y1 * P + c];
+            if (dxv > 0) {
+                nphx = phx_o[xrow0 + (size_t)y1 * dn];
+                nchx = chx[xrow0 + (size_t)y1 * dn];
+            }
+            if (ytask) {
+                nchy = chy[yrow0 + (size_t)(y1 - yb0) * P];
+                if (qshift) nq = phy_o[yrow0 + (size_t)(rc1 + 3) * P];
+            }
+        }
+        if (shift) npw = pc[(size_t)(yc1 + 3) * P + c];
+
+        const int wr = y - yc + 3;
+        float prow = w[3];
+        if (wr == 0) prow = w[0];
+        else if (wr == 1) prow = w[1];
+        else if (wr == 2) prow = w[2];
+        else if (wr == 4) prow = w[4];
+        else if (wr == 5) prow = w[5];
+        else if (wr == 6) prow = w[6];
+
+        // x derivatives of every lane's column, then of the clamped column
+        const float l1 = from_left(prow), l2 = from_left(l1), l3 = from_left(l2);
+        const float r1 = from_right(prow), r2 = from_right(r1), r3 = from_right(r2);
+        const float d2x = lane_read(dd2(l3, l2, l1, prow, r1, r2, r3), xsrc);
+        const float d1x = lane_read(dd1(l3, l2, l1, r1, r2, r3), xsrc);
+        const float f1 = from_left(phxr), f2 = from_left(f1), f3 = from_left(f2);
+        const float g1 = from_right(phxr), g2 = from_right(g1), g3 = from_right(g2);
+        const float dphx = lane_read(dd1(f3, f2, f1, g1, g2, g3), bsrc);
+
+        const int ry = y - t.ya;
+        const bool has_inj = (imask >> ry) & 1ull;
+        int ib = 0, ie = 0;
+        if (has_inj) {
+            ib = rowtab[ioff + ry];
+            ie = rowtab[ioff + ry + 1];
+            for (int e = ib + lane; e < ie; e += 64) {
+                const i2v en = enttab[e];
+                L.src[en.x < XL ? en.x : en.x - (NX - 64)] = itr_t[en.y];
+            }
+            __syncthreads();
+        }
+
+        const int dyv = ytask ? (yband == 0 ? dn - y : y - (NY - dn) + 1) : 0;
+        const int rb = y - yb0;
+        const bool model_row = y >= dn && y < dn + prm.ny;
+
+        float pp = 0.0f;
+        if (in) {
+            const float d2y = dd2(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+            const float base = (-pmr) + 2.0f * prow;
+            const float lap = d2x + d2y;
+            if (dxv == 0 && dyv == 0) {
+                float qq = lap;
+                if (has_inj) qq = qq + L.src[lane];
+                pp = base + qq * c2r;
+            } else {
+                const int m = dxv > dyv ? dxv : dyv;
+                const float A = prm.fA[m], B = prm.fB[m];
+                float propx = 0.0f, propy = 0.0f;
+                if (dxv > 0) {
+                    const float phn = A * phxr + B * d1x;
+                    const float chn = A * chxr + B * (d2x + dphx);
+                    phx_n[xrow0 + (size_t)y * dn] = phn;
+                    chx[xrow0 + (size_t)y * dn] = chn;
+                    propx = dphx + chn;
+                }
+                if (dyv > 0) {
+                    const float d1y = dd1(w[0], w[1], w[2], w[4], w[5], w[6]);
+                    const int qr = rb - rc + 3;
+                    float qown = q[3];
+                    if (qr == 0) qown = q[0];
+                    else if (qr == 1) qown = q[1];
+                    else if (qr == 2) qown = q[2];
+                    else if (qr == 4) qown = q[4];
+                    else if (qr == 5) qown = q[5];
+                    else if (qr == 6) qown = q[6];
+                    const float dph = dd1(q[0], q[1], q[2], q[4], q[5], q[6]);
+                    const float phn = A * qown + B * d1y;
+                    const float chn = A * chyr + B * (d2y + dph);
+                    phy_n[yrow0 + (size_t)rb * P] = phn;
+                    chy[yrow0 + (size_t)rb * P] = chn;
+                    propy = dph + chn;
+                }
+                if (dxv > 0 && dyv > 0) pp = (base + c2r * (lap + propx)) + c2r * propy;
+                else if (dxv > 0) pp = base + c2r * (lap + propx);
+                else pp = base + c2r * (lap + propy);
+            }
+            pn[(size_t)y * P + c] = pp;
+
+            if (prm.snap_op != 0 && model_row && model_col) {
+                const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
+                if (prm.snap_op == 1) {
+                    const float f = ((pp - pmr) * 0.5f) / prm.dt;
+                    snap_store(snap_slot, mi, f, prm.snap_kind, snap_inv);
+                } else {
+                    const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
+                    const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
+                    grad[mi] = grad[mi] + k2;
+                }
+            }
+        }
+
+        if ((rmask >> ry) & 1ull) {
+            L.out[lane] = pp;
+            __syncthreads();
+            const int rbeg = rowtab[t.rec_off + ry], rend = rowtab[t.rec_off + ry + 1];
+            for (int e = rbeg + lane; e < rend; e += 64) {
+                const i2v en = enttab[e];
+                rec_t[en.y] = L.out[en.x < XL ? en.x : en.x - (NX - 64)];
+            }
+            __syncthreads();
+        }
+        if (has_inj) {
+            __syncthreads();
+            for (int e = ib + lane; e < ie; e += 64) {
+                const int x = enttab[e].x;
+                L.src[x < XL ? x : x - (NX - 64)] = 0.0f;
+            }
+            __syncthreads();
+        }
+
+        pmr = npm;
+        c2r = nc2;
+        phxr = nphx;
+        chxr = nchx;
+        chyr = nchy;
+        if (shift) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) w[r] = w[r + 1];
+            w[6] = npw;
+            yc = yc1;
+        }
+        if (qshift) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) q[r] = q[r + 1];
+            q[6] = nq;
+            rc = rc1;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Interior kernel: all cells of the task are model cells at least acc rows
 // and columns away from the frame, so no clamping and no CPML terms apply.
@@ -372,7 +642,10 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
 __device__ __forceinline__ void interior_task(const StepParams& prm, const Task& t, InteriorLds& L) {
     float4* lds_src = L.src;
     float4* lds_out = L.out;
-    const ShotDev& s = prm.shots[t.shot];
+    const GMEM ShotDev& s = gmem(prm.shots)[t.shot];
+    const GMEM int* __restrict__ rowtab = gmem(prm.rows);
+    const GMEM i2v* __restrict__ enttab = reinterpret_cast<const GMEM i2v*>(gmem(prm.ents));
+    GMEM float* __restrict__ grad = gmem(s.grad);
 
     const int lane = threadIdx.x;
     const int P = s.P, dn = prm.dn, Pm = prm.Pm;
@@ -382,31 +655,31 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
     const bool cl = lane >= 1 && lane < 63 && xl < t.xe;  // holds a column the tile writes
     const bool full = cl && xl + 4 <= t.xe;
     const int cur = prm.k & 1;
-    const float* __restrict__ pc = s.p[cur];
-    float* __restrict__ pn = s.p[cur ^ 1];
-    const float* __restrict__ c2i = s.c2i;
+    const GMEM float* __restrict__ pc = gmem(s.p[cur]);
+    GMEM float* __restrict__ pn = gmem(s.p[cur ^ 1]);
+    const GMEM float* __restrict__ c2i = gmem(s.c2i);
 
     const int mode = prm.mode;
     const unsigned long long imask = mode ? t.inj_mask[1] : t.inj_mask[0];
     const int ioff = mode ? t.inj_off[1] : t.inj_off[0];
-    const float* __restrict__ itr_t = (mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]);
+    const GMEM float* __restrict__ itr_t = gmem((mode ? s.inj_tr[1] : s.inj_tr[0]) + (size_t)(prm.k - 2) * (mode ? s.ntr[1] : s.ntr[0]));
     const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
-    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
+    GMEM float* __restrict__ rec_t = gmem(s.rec_tr + (size_t)(prm.k - 2) * s.nrec);
     const int snap_op = prm.snap_op;
     const int skind = prm.snap_kind;
     float snap_scale = 1.0f, snap_inv = 1.0f;
-    void* snap_slot = nullptr;
+    GMEM void* snap_slot = nullptr;
     if (snap_op != 0) {
         snap_slot = slot_base(s, prm);
         if (skind != SNAP_F32 && s.snap_scale) {
-            snap_scale = s.snap_scale[prm.slot];
+            snap_scale = gmem(s.snap_scale)[prm.slot];
             snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
         }
     }
     const int mcol = xl - dn + s.moff;      // model storage column of this lane's first cell
-    const float* lanes_p = pc + xl;
-    float* lanes_pn = pn + xl;
-    const float* lanes_c2 = c2i + xl;
+    const GMEM float* lanes_p = pc + xl;
+    GMEM float* lanes_pn = pn + xl;
+    const GMEM float* lanes_c2 = c2i + xl;
 
     lds_src[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
@@ -449,17 +722,17 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
         int ib = 0, ie = 0;
         float4 src = z4;
         if (has_inj) {
-            ib = prm.rows[ioff + ry];
-            ie = prm.rows[ioff + ry + 1];
+            ib = rowtab[ioff + ry];
+            ie = rowtab[ioff + ry + 1];
             float* ls = reinterpret_cast<float*>(lds_src);
             for (int e = ib + lane; e < ie; e += 64) {
-                const int2 en = prm.ents[e];
+                const i2v en = enttab[e];
                 ls[en.x - x0] = itr_t[en.y];
             }
             __syncthreads();
             src = lds_src[lane];
             __syncthreads();
-            for (int e = ib + lane; e < ie; e += 64) ls[prm.ents[e].x - x0] = 0.0f;
+            for (int e = ib + lane; e < ie; e += 64) ls[enttab[e].x - x0] = 0.0f;
             __syncthreads();
         }
 
@@ -481,7 +754,7 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
             pp.w = ((-pm.w) + 2.0f * c.w) + lap3 * cc.w;
         }
 
-        float* dst = lanes_pn + (size_t)y * P;
+        GMEM float* dst = lanes_pn + (size_t)y * P;
         if (full) {
             st4(dst, pp);
         } else if (cl) {
@@ -501,7 +774,7 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
                     f.y = ((pp.y - pm.y) * 0.5f) / prm.dt;
                     f.z = ((pp.z - pm.z) * 0.5f) / prm.dt;
                     f.w = ((pp.w - pm.w) * 0.5f) / prm.dt;
-                    st4(static_cast<float*>(snap_slot) + mi, f);
+                    st4(static_cast<GMEM float*>(snap_slot) + mi, f);
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
@@ -509,9 +782,9 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
                             snap_store(snap_slot, mi + i, ((ppv[i] - pmv[i]) * 0.5f) / prm.dt, skind, snap_inv);
                 }
             } else {
-                float* g = s.grad + mi;
+                GMEM float* g = grad + mi;
                 if (full && skind == SNAP_F32) {
-                    const float4 f = ld4(static_cast<const float*>(snap_slot) + mi);
+                    const float4 f = ld4(static_cast<const GMEM float*>(snap_slot) + mi);
                     float4 gv = ld4(g);
                     gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
                     gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
@@ -533,9 +806,9 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
             lds_out[lane] = pp;
             __syncthreads();
             const float* lo = reinterpret_cast<const float*>(lds_out);
-            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            const int rbeg = rowtab[t.rec_off + ry], rend = rowtab[t.rec_off + ry + 1];
             for (int e = rbeg + lane; e < rend; e += 64) {
-                const int2 en = prm.ents[e];
+                const i2v en = enttab[e];
                 rec_t[en.y] = lo[en.x - x0];
             }
             __syncthreads();
@@ -555,18 +828,38 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
 }
 
 // ---------------------------------------------------------------------------
-// Y-band task: rows 3..dn-4 of the top or bottom CPML band (counted from the
-// band's first row), columns strictly between the x bands.  There the update
-// has only the y-CPML term, no derivative is clamped (neither p's nor the
-// band-local one of phi_y) and the CPML factor depends on the row only.
-// Two columns per lane (lanes 0,1 and 62,63 only load halo), x stencil
-// through DPP, p and phi_y in 8-slot register windows rotated by unrolling.
-__device__ __forceinline__ float2 ld2(const float* p) { return *reinterpret_cast<const float2*>(p); }
-__device__ __forceinline__ void st2(float* p, float2 v) { *reinterpret_cast<float2*>(p) = v; }
+// Y-band task: rows [ya, yb) of the top or bottom CPML band, columns strictly
+// between the x bands.  There the update has only the y-CPML term and the
+// CPML factor depends on the row only.  Two columns per lane (lanes 0,1 and
+// 62,63 only load halo), x stencil through DPP, p and phi_y in 8-slot
+// register windows rotated by unrolling, centred on the row being updated.
+// Clamped derivatives (solver_space_derivs.f90: the outer acc lines copy the
+// nearest computed line; solver_PML_matrices.f90: the same within the band
+// for the local derivative of phi_y) are formed at the clamp line before the
+// sweep, for the tasks whose rows need them.
+__device__ __forceinline__ float2 ld2(const GMEM float* p) {
+    const f2v v = *reinterpret_cast<const GMEM f2v*>(p);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(GMEM float* p, float2 v) {
+    const f2v w = {v.x, v.y};
+    *reinterpret_cast<GMEM f2v*>(p) = w;
+}
+
+__device__ __forceinline__ float2 dd2v(const float2& m3, const float2& m2, const float2& m1, const float2& c,
+                                       const float2& p1, const float2& p2, const float2& p3) {
+    return make_float2(dd2(m3.x, m2.x, m1.x, c.x, p1.x, p2.x, p3.x), dd2(m3.y, m2.y, m1.y, c.y, p1.y, p2.y, p3.y));
+}
+__device__ __forceinline__ float2 dd1v(const float2& m3, const float2& m2, const float2& m1, const float2& p1,
+                                       const float2& p2, const float2& p3) {
+    return make_float2(dd1(m3.x, m2.x, m1.x, p1.x, p2.x, p3.x), dd1(m3.y, m2.y, m1.y, p1.y, p2.y, p3.y));
+}
 
 __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t, InteriorLds& L) {
     float2* lds_out = reinterpret_cast<float2*>(L.out);
-    const ShotDev& s = prm.shots[t.shot];
+    const GMEM ShotDev& s = gmem(prm.shots)[t.shot];
+    const GMEM int* __restrict__ rowtab = gmem(prm.rows);
+    const GMEM i2v* __restrict__ enttab = reinterpret_cast<const GMEM i2v*>(gmem(prm.ents));
 
     const int lane = threadIdx.x;
     const int P = s.P, dn = prm.dn, NY = prm.NY;
@@ -576,34 +869,69 @@ __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t,
     const bool cl = lane >= 2 && lane < 62 && xl < t.xe;
     const bool full = cl && xl + 2 <= t.xe;
     const int cur = prm.k & 1;
-    const float* __restrict__ pc = s.p[cur];
-    float* __restrict__ pn = s.p[cur ^ 1];
+    const GMEM float* __restrict__ pc = gmem(s.p[cur]);
+    GMEM float* __restrict__ pn = gmem(s.p[cur ^ 1]);
     const int yband = t.ya < dn ? 0 : 1;
-    const int yb0 = yband == 0 ? 0 : NY - dn;
+    const int yb0 = yband == 0 ? 0 : NY - dn;                       // first row of the band
     const size_t brow0 = (size_t)(yband * dn) * P + xl;            // band row r at brow0 + r*P
-    const float* __restrict__ qo = s.phy[cur] + brow0;
-    float* __restrict__ qn = s.phy[cur ^ 1] + brow0;
-    float* __restrict__ chy = s.chy + brow0;
-    const float* lanes_p = pc + xl;
-    float* lanes_pn = pn + xl;
-    const float* lanes_c2 = s.c2 + xl;
+    const GMEM float* __restrict__ qo = gmem(s.phy[cur] + brow0);
+    GMEM float* __restrict__ qn = gmem(s.phy[cur ^ 1] + brow0);
+    GMEM float* __restrict__ chy = gmem(s.chy + brow0);
+    const GMEM float* lanes_p = pc + xl;
+    GMEM float* lanes_pn = pn + xl;
+    const GMEM float* lanes_c2 = gmem(s.c2) + xl;
     const unsigned long long rmask = prm.record ? t.rec_mask : 0ull;
-    float* __restrict__ rec_t = s.rec_tr + (size_t)(prm.k - 2) * s.nrec;
+    GMEM float* __restrict__ rec_t = gmem(s.rec_tr + (size_t)(prm.k - 2) * s.nrec);
+    const int ylo = ACC, yhi = NY - 1 - ACC;                        // computed lines of p
+    const int rlo = ACC, rhi = dn - 1 - ACC;                        // computed lines of phi_y in the band
+    const int ra = t.ya - yb0, rz = t.yb - 1 - yb0;                 // first and last band row of the task
 
     const float2 z2 = make_float2(0.0f, 0.0f);
+    // derivatives at the clamp lines the task's rows take: phi_y at band rows
+    // rlo / rhi, p at grid rows ylo (top band) / yhi (bottom band)
+    float2 dph_lo = z2, d2y_lo = z2, d1y_lo = z2, dph_hi = z2, d2y_hi = z2, d1y_hi = z2;
+    if (cl) {
+        float2 u[7];
+        if (ra < rlo) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) u[r] = ld2(qo + (size_t)(rlo - 3 + r) * P);
+            dph_lo = dd1v(u[0], u[1], u[2], u[4], u[5], u[6]);
+        }
+        if (rz > rhi) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) u[r] = ld2(qo + (size_t)(rhi - 3 + r) * P);
+            dph_hi = dd1v(u[0], u[1], u[2], u[4], u[5], u[6]);
+        }
+        if (t.ya < ylo) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) u[r] = ld2(lanes_p + (size_t)(ylo - 3 + r) * P);
+            d2y_lo = dd2v(u[0], u[1], u[2], u[3], u[4], u[5], u[6]);
+            d1y_lo = dd1v(u[0], u[1], u[2], u[4], u[5], u[6]);
+        }
+        if (t.yb - 1 > yhi) {
+#pragma unroll
+            for (int r = 0; r < 7; ++r) u[r] = ld2(lanes_p + (size_t)(yhi - 3 + r) * P);
+            d2y_hi = dd2v(u[0], u[1], u[2], u[3], u[4], u[5], u[6]);
+            d1y_hi = dd1v(u[0], u[1], u[2], u[4], u[5], u[6]);
+        }
+    }
+
+    // windows centred on the first row; lines outside the grid (band) read as 0
     float2 S[8], Q[8];
-    const int rb_a = t.ya - yb0;
 #pragma unroll
     for (int r = 0; r < 7; ++r) {
-        S[r] = ld ? ld2(lanes_p + (size_t)(t.ya - 3 + r) * P) : z2;
-        Q[r] = cl ? ld2(qo + (size_t)(rb_a - 3 + r) * P) : z2;
+        const int y = t.ya - 3 + r, rb = ra - 3 + r;
+        if (ld && y >= 0 && y < NY) S[r] = ld2(lanes_p + (size_t)y * P);
+        else S[r] = z2;
+        if (cl && rb >= 0 && rb < dn) Q[r] = ld2(qo + (size_t)rb * P);
+        else Q[r] = z2;
     }
     S[7] = z2;
     Q[7] = z2;
     float2 PM[2], C2[2], CH[2];
     PM[0] = cl ? ld2(lanes_pn + (size_t)t.ya * P) : z2;
     C2[0] = cl ? ld2(lanes_c2 + (size_t)t.ya * P) : z2;
-    CH[0] = cl ? ld2(chy + (size_t)rb_a * P) : z2;
+    CH[0] = cl ? ld2(chy + (size_t)ra * P) : z2;
     PM[1] = C2[1] = CH[1] = z2;
 
     auto row = [&](int y0, auto Jc) -> bool {
@@ -612,9 +940,12 @@ __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t,
         if (y >= t.yb) return false;
         const int rb = y - yb0;
         if (y + 1 < t.yb) {
-            if (ld) S[(j + 7) & 7] = ld2(lanes_p + (size_t)(y + 4) * P);
+            // (if/else rather than ?: so the zero is not taken by address)
+            if (ld && y + 4 < NY) S[(j + 7) & 7] = ld2(lanes_p + (size_t)(y + 4) * P);
+            else S[(j + 7) & 7] = z2;
+            if (cl && rb + 4 < dn) Q[(j + 7) & 7] = ld2(qo + (size_t)(rb + 4) * P);
+            else Q[(j + 7) & 7] = z2;
             if (cl) {
-                Q[(j + 7) & 7] = ld2(qo + (size_t)(rb + 4) * P);
                 PM[(j + 1) & 1] = ld2(lanes_pn + (size_t)(y + 1) * P);
                 C2[(j + 1) & 1] = ld2(lanes_c2 + (size_t)(y + 1) * P);
                 CH[(j + 1) & 1] = ld2(chy + (size_t)(rb + 1) * P);
@@ -623,12 +954,20 @@ __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t,
         const float2 m3 = S[j & 7], m2 = S[(j + 1) & 7], m1 = S[(j + 2) & 7], c = S[(j + 3) & 7];
         const float2 p1 = S[(j + 4) & 7], p2 = S[(j + 5) & 7], p3 = S[(j + 6) & 7];
         const float2 q3 = Q[j & 7], q2 = Q[(j + 1) & 7], q1 = Q[(j + 2) & 7], qc = Q[(j + 3) & 7];
-        const float2 r1 = Q[(j + 4) & 7], r2 = Q[(j + 5) & 7], r3 = Q[(j + 6) & 7];
+        const float2 u1 = Q[(j + 4) & 7], u2 = Q[(j + 5) & 7], u3 = Q[(j + 6) & 7];
         const float2 pm = PM[j & 1], cc = C2[j & 1], ch = CH[j & 1];
 
         // columns x-3 .. x+4 around this lane's pair (x, x+1)
         const float lm1 = from_left(c.y), lm2 = from_left(c.x), lm3 = from_left(lm1);
         const float rp2 = from_right(c.x), rp3 = from_right(c.y), rp4 = from_right(rp2);
+
+        float2 d2y = dd2v(m3, m2, m1, c, p1, p2, p3);
+        float2 d1y = dd1v(m3, m2, m1, p1, p2, p3);
+        float2 dph = dd1v(q3, q2, q1, u1, u2, u3);
+        if (y < ylo) { d2y = d2y_lo; d1y = d1y_lo; }
+        else if (y > yhi) { d2y = d2y_hi; d1y = d1y_hi; }
+        if (rb < rlo) dph = dph_lo;
+        else if (rb > rhi) dph = dph_hi;
 
         const int dyv = yband == 0 ? dn - y : y - (NY - dn) + 1;
         const float A = prm.fA[dyv], B = prm.fB[dyv];
@@ -636,21 +975,18 @@ __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t,
 #define FWI_YBAND_CELL(F, D2X)                                                                  \
         {                                                                                       \
             const float d2x = D2X;                                                              \
-            const float d2y = dd2(m3.F, m2.F, m1.F, c.F, p1.F, p2.F, p3.F);                     \
-            const float d1y = dd1(m3.F, m2.F, m1.F, p1.F, p2.F, p3.F);                          \
-            const float dph = dd1(q3.F, q2.F, q1.F, r1.F, r2.F, r3.F);                          \
-            phn.F = A * qc.F + B * d1y;                                                         \
-            chn.F = A * ch.F + B * (d2y + dph);                                                 \
-            const float propy = dph + chn.F;                                                    \
-            pp.F = ((-pm.F) + 2.0f * c.F) + cc.F * ((d2x + d2y) + propy);                       \
+            phn.F = A * qc.F + B * d1y.F;                                                       \
+            chn.F = A * ch.F + B * (d2y.F + dph.F);                                             \
+            const float propy = dph.F + chn.F;                                                  \
+            pp.F = ((-pm.F) + 2.0f * c.F) + cc.F * ((d2x + d2y.F) + propy);                     \
         }
         FWI_YBAND_CELL(x, dd2(lm3, lm2, lm1, c.x, c.y, rp2, rp3))
         FWI_YBAND_CELL(y, dd2(lm2, lm1, c.x, c.y, rp2, rp3, rp4))
 #undef FWI_YBAND_CELL
 
-        float* dst = lanes_pn + (size_t)y * P;
-        float* dq = qn + (size_t)rb * P;
-        float* dc = chy + (size_t)rb * P;
+        GMEM float* dst = lanes_pn + (size_t)y * P;
+        GMEM float* dq = qn + (size_t)rb * P;
+        GMEM float* dc = chy + (size_t)rb * P;
         if (full) {
             st2(dst, pp);
             st2(dq, phn);
@@ -666,9 +1002,9 @@ __device__ __forceinline__ void yband_task(const StepParams& prm, const Task& t,
             lds_out[lane] = pp;
             __syncthreads();
             const float* lo = reinterpret_cast<const float*>(lds_out);
-            const int rbeg = prm.rows[t.rec_off + ry], rend = prm.rows[t.rec_off + ry + 1];
+            const int rbeg = rowtab[t.rec_off + ry], rend = rowtab[t.rec_off + ry + 1];
             for (int e = rbeg + lane; e < rend; e += 64) {
-                const int2 en = prm.ents[e];
+                const i2v en = enttab[e];
                 rec_t[en.y] = lo[en.x - x0];
             }
             __syncthreads();
@@ -693,6 +1029,7 @@ __global__ __launch_bounds__(64) void step_kernel(const StepParams prm) {
     __shared__ StepLds lds;
     const Task t = prm.tasks[task_index()];
     if (t.kind == TASK_FRAME) frame_task(prm, t, lds.f);
+    else if (t.kind == TASK_XPAIR) xpair_task(prm, t, lds.f);
     else if (t.kind == TASK_INTERIOR) interior_task(prm, t, lds.i);
     else if (t.kind == TASK_YBAND) yband_task(prm, t, lds.i);
 }

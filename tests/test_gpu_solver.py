@@ -173,3 +173,42 @@ def test_batched_windows_vs_oracle(rows):
     for i in range(nx):
         g_o[:water[i], i] = 0.0
     assert np.allclose(grad, g_o, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("dpml,nx,fs", [
+    (2, 60, 1),     # dn = 8: thinnest frame
+    (3, 60, 0),
+    (20, 8, 0),     # padded width 60 < 64: x bands through the general path
+    (22, 60, 0),    # dn = 28: widest frame whose x bands pair in one wavefront
+    (23, 60, 1),    # dn = 29: x bands through the general path
+    (50, 61, 0),
+])
+def test_frame_paths_vs_oracle(dpml, nx, fs):
+    """Every variant of the frame decomposition: receivers spread over the whole
+    padded grid (corners, bands, clamped outer lines), sources in the model
+    and in the frame; forward traces, snapshots and the adjoint kernel are
+    bit-exact against the oracle."""
+    rng = np.random.default_rng(dpml * 100 + nx)
+    ny, nt, ss = 30, 120, 10
+    dn = dpml + 6
+    c = (1500 + 2000 * rng.random((ny, nx))).astype(np.float32)
+    nxs = np.array([1 + nx // 2, 1, nx, -dn + 2])
+    nys = np.array([1 + ny // 3, ny, 1, 3])
+    S = np.stack([synth.ricker(12.0, nt, 0.002) * (1 + i) for i in range(len(nxs))], 1).astype(np.float32)
+    nxr = np.concatenate([rng.integers(1 - dn, nx + dn + 1, 40), [1 - dn, nx + dn, 1 - dn, nx + dn, 1, nx]])
+    nyr = np.concatenate([rng.integers(1 - dn, ny + dn + 1, 40), [1 - dn, 1 - dn, ny + dn, ny + dn, 1, ny]])
+    d, dt = np.float32(20.0), np.float32(0.0015)
+    rec, F, _ = fwi_amd.solver_acoustic_SS(1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nt // ss, nt, dpml=dpml,
+                                           free_surface=fs)
+    O = rb.oracle_solver("f32", 1, nxs, nys, nxr, nyr, S, None, d, dt, c, ss, nt // ss, nt, dpml=dpml,
+                         free_surface=fs)
+    assert np.abs(O.rec).max() > 0
+    assert np.array_equal(rec, O.rec), f"rec max diff {np.abs(rec - O.rec).max()}"
+    assert np.array_equal(F, O.fields)
+    adj = np.asfortranarray(rng.standard_normal((nt, len(nxr))).astype(np.float32))
+    _, _, k = fwi_amd.solver_acoustic_SS(2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nt // ss, nt,
+                                         Fields=O.fields.copy(order="F"), dpml=dpml, free_surface=fs)
+    O2 = rb.oracle_solver("f32", 2, nxs, nys, nxr, nyr, S, adj, d, dt, c, ss, nt // ss, nt, fields=O.fields,
+                          dpml=dpml, free_surface=fs)
+    assert np.abs(O2.kernel).max() > 0
+    assert np.array_equal(k, O2.kernel), f"kernel max diff {np.abs(k - O2.kernel).max()}"
