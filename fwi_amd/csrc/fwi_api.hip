@@ -138,6 +138,28 @@ struct fwi_ctx {
     double* d_shot_misfit = nullptr;
     double* d_grad = nullptr;
     int* d_water = nullptr;
+    // Snapshot schedule.  seg == 0: every slot 1..nt_store is stored by the
+    // forward sweep.  seg == S > 0: two-level checkpointing -- the forward
+    // sweep keeps the full propagation state every S snapshot intervals and
+    // stores only the last segment's slots; before the adjoint consumes an
+    // earlier segment, it is recomputed from its checkpoint into the S-slot
+    // segment buffer, with the adjoint running on its own state.  Slot 0 of
+    // the buffer stays zero (the reference's unused Fields slot 0).
+    int seg = 0;
+    int nslots = 0;                       // slots allocated per shot after the zero slot
+    bool storage_ready = false;
+    bool last_segment_resident = false;   // the forward sweep left the last segment in the buffer
+    Arena astate, ckpt, stage;
+    fwi::ShotDev* d_desc_adj = nullptr;   // descriptors on the adjoint state (== d_desc without checkpoints)
+    fwi::QuantShot* d_quant = nullptr;
+    unsigned int* d_snapmax = nullptr;    // [shot][slot] max |value| bits (16-bit kinds)
+    size_t nsnapmax = 0;
+    size_t max_slot_elems = 0;
+    std::vector<fwi::ShotDev> desc;       // host copy of the forward descriptors
+    // Time-step sweeps are captured once as HIP graphs and replayed.
+    bool use_graphs = true;
+    std::map<std::string, hipGraphExec_t> graphs;
+
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;   // step-kernel spans not yet read
@@ -147,12 +169,24 @@ struct fwi_ctx {
 
     ~fwi_ctx() {
         release();
+        drop_graphs();
         for (auto& e : ev_pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         if (stream) (void)hipStreamDestroy(stream);
     }
     void release() {
-        state.release(); stat.release(); traces.release(); snaps.release(); grads.release(); misc.release();
+        release_storage();
+        state.release(); stat.release(); traces.release(); grads.release(); misc.release();
         have_shots = have_model = have_synth = have_snaps = false;
+    }
+    void release_storage() {
+        drop_graphs();
+        snaps.release(); astate.release(); ckpt.release(); stage.release();
+        storage_ready = false;
+        have_snaps = false;
+    }
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
     }
     size_t snap_elem() const { return cfg.snap_kind == fwi::SNAP_F32 ? 4 : 2; }
 };
@@ -330,7 +364,6 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         for (int b = 0; b < 2; ++b) s.o_phy[b] = c->state.add(yb);
         s.o_chy = c->state.add(yb);
         s.o_c2 = c->stat.add(field_bytes);
-        s.o_snap = c->snaps.add((size_t)(c->nt_store + 1) * ny * s.Ws * c->snap_elem());
         s.o_grad = c->grads.add((size_t)ny * s.Ws * sizeof(float));
     }
     c->Pm = (int)align_up(cf.nx + 8, 64);
@@ -402,6 +435,8 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     const int nblk = 64;
     c->nblk_misfit = nblk;
     const size_t o_desc = c->misc.add(nshots * sizeof(fwi::ShotDev));
+    const size_t o_desc_adj = c->misc.add(nshots * sizeof(fwi::ShotDev));
+    const size_t o_quant = c->misc.add(nshots * sizeof(fwi::QuantShot));
     const size_t o_tasks = c->misc.add(std::max<size_t>(1, tasks.size()) * sizeof(fwi::Task));
     const size_t o_vp = c->misc.add((size_t)cf.ny * cf.nx * sizeof(float));
     const size_t o_c2sh = c->misc.add(nshots * sizeof(fwi::C2Shot));
@@ -417,12 +452,13 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     CHK(c->state.alloc());
     CHK(c->stat.alloc());
     CHK(c->traces.alloc());
-    CHK(c->snaps.alloc());
     CHK(c->grads.alloc());
     CHK(c->misc.alloc());
     HIPCHK(hipMemset(c->stat.base, 0, c->stat.size));
 
     c->d_desc = c->misc.at<fwi::ShotDev>(o_desc);
+    c->d_desc_adj = c->misc.at<fwi::ShotDev>(o_desc_adj);
+    c->d_quant = c->misc.at<fwi::QuantShot>(o_quant);
     c->d_tasks = c->misc.at<fwi::Task>(o_tasks);
     c->d_vp = c->misc.at<float>(o_vp);
     c->d_c2shots = c->misc.at<fwi::C2Shot>(o_c2sh);
@@ -469,8 +505,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         d.ntr[0] = (int)s.sx.size();
         d.ntr[1] = (int)s.rx.size();
         d.rec_tr = c->traces.at<float>(s.o_rec);
-        d.snap = c->snaps.at<void>(s.o_snap);
+        d.snap = nullptr;                 // snapshot storage: attach_storage()
         d.snap_scale = nullptr;
+        d.stage = nullptr;
+        d.snap_max = nullptr;
         d.grad = c->grads.at<float>(s.o_grad);
         d.snap_stride = (size_t)cf.ny * s.Ws;
         c2s[si] = fwi::C2Shot{const_cast<float*>(d.c2), s.NX, s.P, s.W, s.x0};
@@ -478,7 +516,11 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         rs[si] = fwi::ResShot{s.toff, s.roff, (int)s.rx.size()};
         gs[si] = fwi::GradShot{d.grad, s.x0, s.W, s.Ws, s.moff};
     }
+    c->desc = desc;
+    c->max_slot_elems = 0;
+    for (auto& s : c->shots) c->max_slot_elems = std::max(c->max_slot_elems, (size_t)ny * s.Ws);
     HIPCHK(hipMemcpy(c->d_desc, desc.data(), nshots * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_desc_adj, desc.data(), nshots * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIPCHK(hipMemcpy(c->d_tasks, tasks.data(), tasks.size() * sizeof(fwi::Task), hipMemcpyHostToDevice));
     if (!rows.empty()) HIPCHK(hipMemcpy(c->d_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ents.empty()) HIPCHK(hipMemcpy(c->d_ents, ents.data(), ents.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -487,21 +529,141 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     HIPCHK(hipMemcpy(c->d_resshots, rs.data(), nshots * sizeof(fwi::ResShot), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_gradshots, gs.data(), nshots * sizeof(fwi::GradShot), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->traces.base, 0, c->traces.size));
-    HIPCHK(hipMemset(c->snaps.base, 0, c->snaps.size));
     HIPCHK(hipMemset(c->grads.base, 0, c->grads.size));
     c->have_shots = true;
     return 0;
 }
 
-// Time loop: k = 2..nt as in solver_acoustic_SS.f90:313.
-// mode 0 forward (record receivers; store snapshots if store), 1 adjoint
-// (inject adjoint sources at receivers; imaging condition).
-int run(fwi_ctx* c, int mode, bool store) {
+// ---------------------------------------------------------------------------
+// Snapshot storage and the time-step sweeps.
+
+// Segments of the checkpointed schedule: segment j holds slots
+// j*seg+1 .. min((j+1)*seg, nt_store-1) (the adjoint never reads slot
+// nt_store, and slot 0 is the zero slot).
+int num_segments(const fwi_ctx* c, int seg) {
+    const int n = c->nt_store - 1;
+    return (seg > 0 && n > 0) ? (n + seg - 1) / seg : 0;
+}
+int slots_for(const fwi_ctx* c, int seg) { return seg > 0 ? std::min(seg, std::max(c->nt_store - 1, 0)) : c->nt_store; }
+
+// Device bytes of the snapshot storage of a schedule: slots (+ zero slot),
+// FP32 staging for the 16-bit kinds, checkpoints and the adjoint state.
+size_t storage_bytes(const fwi_ctx* c, int seg) {
+    const size_t ny = c->cfg.ny;
+    const bool q16 = c->cfg.snap_kind != fwi::SNAP_F32;
+    const size_t nsl = slots_for(c, seg) + 1;
+    size_t b = 0;
+    for (const Shot& s : c->shots) {
+        b += align_up(nsl * ny * s.Ws * c->snap_elem(), 256);
+        if (q16) b += align_up(ny * s.Ws * sizeof(float), 256) + align_up(nsl * sizeof(float), 256);
+    }
+    if (q16) b += align_up(c->shots.size() * nsl * sizeof(unsigned int), 256);
+    if (seg > 0) b += (size_t)std::max(num_segments(c, seg) - 1, 0) * c->state.size + c->state.size;
+    return b;
+}
+
+// Allocates the snapshot storage of the current schedule and points the
+// shot descriptors at it (the adjoint descriptors at the adjoint state).
+int attach_storage(fwi_ctx* c) {
+    if (c->storage_ready) return 0;
+    const int ns = (int)c->shots.size();
+    const size_t ny = c->cfg.ny;
+    const bool q16 = c->cfg.snap_kind != fwi::SNAP_F32;
+    c->nslots = slots_for(c, c->seg);
+    const size_t nsl = c->nslots + 1;
+    std::vector<size_t> o_stage(ns), o_scale(ns);
+    for (Shot& s : c->shots) s.o_snap = c->snaps.add(nsl * ny * s.Ws * c->snap_elem());
+    size_t o_max = 0;
+    if (q16) {
+        for (int i = 0; i < ns; ++i) {
+            o_stage[i] = c->stage.add(ny * c->shots[i].Ws * sizeof(float));
+            o_scale[i] = c->stage.add(nsl * sizeof(float));
+        }
+        o_max = c->stage.add(ns * nsl * sizeof(unsigned int));
+        c->nsnapmax = ns * nsl;
+    }
+    const int nseg = num_segments(c, c->seg);
+    if (c->seg > 0) {
+        c->astate.add(c->state.size);
+        if (nseg > 1) c->ckpt.add((size_t)(nseg - 1) * c->state.size);
+    }
+    CHK(c->snaps.alloc());
+    CHK(c->stage.alloc());
+    CHK(c->astate.alloc());
+    CHK(c->ckpt.alloc());
+    if (c->snaps.size) HIPCHK(hipMemset(c->snaps.base, 0, c->snaps.size));
+    if (c->stage.size) HIPCHK(hipMemset(c->stage.base, 0, c->stage.size));
+    c->d_snapmax = q16 ? c->stage.at<unsigned int>(o_max) : nullptr;
+
+    std::vector<fwi::ShotDev> fwd = c->desc, adj;
+    std::vector<fwi::QuantShot> qs(ns);
+    for (int i = 0; i < ns; ++i) {
+        const Shot& s = c->shots[i];
+        fwi::ShotDev& d = fwd[i];
+        d.snap = c->snaps.at<void>(s.o_snap);
+        if (q16) {
+            d.stage = c->stage.at<float>(o_stage[i]);
+            d.snap_scale = c->stage.at<float>(o_scale[i]);
+            d.snap_max = c->d_snapmax + (size_t)i * nsl;
+            qs[i] = fwi::QuantShot{d.stage, static_cast<unsigned short*>(d.snap), d.snap_scale, d.snap_max,
+                                   d.snap_stride};
+        }
+    }
+    adj = fwd;
+    if (c->seg > 0) {
+        // same layout as the forward state, shifted to the adjoint arena
+        auto mv = [&](float* p) { return reinterpret_cast<float*>(c->astate.base + (reinterpret_cast<char*>(p) - c->state.base)); };
+        for (auto& d : adj) {
+            for (int b = 0; b < 2; ++b) { d.p[b] = mv(d.p[b]); d.phx[b] = mv(d.phx[b]); d.phy[b] = mv(d.phy[b]); }
+            d.chx = mv(d.chx);
+            d.chy = mv(d.chy);
+        }
+    }
+    HIPCHK(hipMemcpy(c->d_desc, fwd.data(), ns * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_desc_adj, adj.data(), ns * sizeof(fwi::ShotDev), hipMemcpyHostToDevice));
+    if (q16) HIPCHK(hipMemcpy(c->d_quant, qs.data(), ns * sizeof(fwi::QuantShot), hipMemcpyHostToDevice));
+    c->storage_ready = true;
+    return 0;
+}
+
+// Runs issue() on the stream, captured once into a graph under `key` and
+// replayed afterwards.  Captured work holds kernels only (no memsets).
+template <class F>
+int in_graph(fwi_ctx* c, const std::string& key, F&& issue) {
+    if (!c->use_graphs) return issue();
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int r = issue();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r != 0) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        HIPCHK(e);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+        it = c->graphs.emplace(key, x).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, c->stream));
+    return 0;
+}
+
+// Steps k0..k1 (Fortran time index, solver_acoustic_SS.f90:313) of one
+// sweep.  Forward (mode 0): slot m = k/ss with lo <= m <= hi is stored
+// into buffer slot m - off; receivers are sampled when record.  Adjoint
+// (mode 1, receivers inject): the step k with k % ss == 0 correlates with
+// slot m = nt_store - k/ss, read from buffer slot m - off (slot 0: the
+// zero slot).
+int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int mode, bool record, int k0, int k1, int lo, int hi,
+                int off) {
     const fwi_config& cf = c->cfg;
-    HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
     fwi::StepParams prm;
     std::memset(&prm, 0, sizeof(prm));
-    prm.shots = c->d_desc;
+    prm.shots = desc;
     prm.tasks = c->d_tasks;
     prm.ntasks = c->ntasks;
     prm.rows = c->d_rows;
@@ -511,60 +673,154 @@ int run(fwi_ctx* c, int mode, bool store) {
     prm.dn = c->dn;
     prm.ny = cf.ny;
     prm.mode = mode;
-    prm.record = mode == 0 ? 1 : 0;
+    prm.record = record ? 1 : 0;
     prm.snap_kind = cf.snap_kind;
     prm.dt = cf.dt;
     prm.ssf = (float)cf.store_snap;
     prm.free_surface = cf.free_surface;
     std::memcpy(prm.fA, c->fA, sizeof(prm.fA));
     std::memcpy(prm.fB, c->fB, sizeof(prm.fB));
-    if (mode == 0) {
-        const size_t ntr = (size_t)c->nrec_total * cf.nt * sizeof(float);
-        if (ntr) HIPCHK(hipMemsetAsync(c->traces.base + c->shots[0].o_rec, 0, ntr, c->stream));
-    }
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (c->timing) {
-        HIPCHK(hipEventCreate(&ev0));
-        HIPCHK(hipEventCreate(&ev1));
-        HIPCHK(hipEventRecord(ev0, c->stream));
-    }
-
-    for (int k = 2; k <= cf.nt; ++k) {
+    const int ss = cf.store_snap;
+    const bool q16 = cf.snap_kind != fwi::SNAP_F32;
+    for (int k = std::max(k0, 2); k <= k1; ++k) {
         prm.k = k;
         prm.snap_op = 0;
         prm.slot = 0;
-        if (k % cf.store_snap == 0) {
-            const int m = k / cf.store_snap;
-            if (mode == 0 && store) {
-                if (m > c->nt_store) return fail("snapshot slot overflow (nt_Store too small)");
-                prm.snap_op = 1;
-                prm.slot = m;
-            } else if (mode == 1) {
-                const int slot = c->nt_store - m;
-                if (slot < 0) return fail("snapshot slot underflow (nt_Store too small)");
+        if (k % ss == 0) {
+            if (mode == 0) {
+                const int m = k / ss;
+                if (m >= lo && m <= hi) {
+                    if (m - off < 1 || m - off > c->nslots) return fail("snapshot slot overflow (nt_Store too small)");
+                    prm.snap_op = 1;
+                    prm.slot = m - off;
+                }
+            } else {
+                const int m = c->nt_store - k / ss;
+                if (m < 0) return fail("snapshot slot underflow (nt_Store too small)");
+                const int b = m == 0 ? 0 : m - off;
+                if (b < 0 || b > c->nslots) return fail("snapshot slot not resident");
                 prm.snap_op = 2;
-                prm.slot = slot;
+                prm.slot = b;
             }
         }
         HIPCHK(fwi::launch_step(prm, c->stream));
+        if (prm.snap_op == 1 && q16)
+            HIPCHK(fwi::launch_quantize(c->d_quant, (int)c->shots.size(), c->max_slot_elems, prm.slot,
+                                        cf.snap_kind == fwi::SNAP_BF16, c->stream));
     }
-    if (c->timing) {
-        HIPCHK(hipEventRecord(ev1, c->stream));
-        c->ev_pending.push_back({ev0, ev1});
-        c->step_launches += cf.nt - 1;
+    return 0;
+}
+
+int reset_snapmax(fwi_ctx* c) {
+    if (c->d_snapmax) HIPCHK(fwi::launch_fill_u32(c->d_snapmax, c->nsnapmax, 0u, c->stream));
+    return 0;
+}
+
+int timing_begin(fwi_ctx* c, hipEvent_t* ev0) {
+    *ev0 = nullptr;
+    if (!c->timing) return 0;
+    HIPCHK(hipEventCreate(ev0));
+    HIPCHK(hipEventRecord(*ev0, c->stream));
+    return 0;
+}
+int timing_end(fwi_ctx* c, hipEvent_t ev0) {
+    if (!c->timing) return 0;
+    hipEvent_t ev1 = nullptr;
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventRecord(ev1, c->stream));
+    c->ev_pending.push_back({ev0, ev1});
+    c->step_launches += c->cfg.nt - 1;
+    return 0;
+}
+
+// Forward sweep k = 2..nt: receivers sampled; with store, snapshots (and the
+// checkpoints of the checkpointed schedule).
+int forward_sweep(fwi_ctx* c, bool store) {
+    const fwi_config& cf = c->cfg;
+    const int nt = cf.nt, ss = cf.store_snap;
+    if (store) CHK(attach_storage(c));
+    HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+    const size_t ntr = (size_t)c->nrec_total * cf.nt * sizeof(float);
+    if (ntr) HIPCHK(hipMemsetAsync(c->traces.base + c->shots[0].o_rec, 0, ntr, c->stream));
+    hipEvent_t ev0;
+    CHK(timing_begin(c, &ev0));
+    if (!store) {
+        CHK(in_graph(c, "forward", [&] { return issue_steps(c, c->d_desc, 0, true, 2, nt, 1, 0, 0); }));
+    } else if (c->seg == 0) {
+        CHK(in_graph(c, "forward_store", [&] {
+            CHK(reset_snapmax(c));
+            return issue_steps(c, c->d_desc, 0, true, 2, nt, 1, c->nslots, 0);
+        }));
+    } else {
+        const int S = c->seg, nseg = num_segments(c, S);
+        CHK(in_graph(c, "forward_checkpoints", [&] {
+            CHK(reset_snapmax(c));
+            int k = 2;
+            for (int j = 1; j < nseg; ++j) {
+                const int kc = j * S * ss;
+                CHK(issue_steps(c, c->d_desc, 0, true, k, kc, 1, 0, 0));
+                HIPCHK(fwi::launch_copy16(c->ckpt.base + (size_t)(j - 1) * c->state.size, c->state.base,
+                                          c->state.size / 16, c->stream));
+                k = kc + 1;
+            }
+            if (nseg == 0) return issue_steps(c, c->d_desc, 0, true, k, nt, 1, 0, 0);
+            const int j = nseg - 1;
+            return issue_steps(c, c->d_desc, 0, true, k, nt, j * S + 1, std::min((j + 1) * S, c->nt_store - 1), j * S);
+        }));
+        c->last_segment_resident = true;
     }
-    if (mode == 0) {
-        // Rec(nt,:) = Rec(nt-1,:)  (solver_acoustic_SS.f90:588)
-        for (auto& s : c->shots) {
-            const size_t n = s.rx.size();
-            if (!n) continue;
-            float* base = c->traces.at<float>(s.o_rec);
-            HIPCHK(hipMemcpyAsync(base + (size_t)(cf.nt - 1) * n, base + (size_t)(cf.nt - 2) * n, n * sizeof(float),
-                                  hipMemcpyDeviceToDevice, c->stream));
+    CHK(timing_end(c, ev0));
+    // Rec(nt,:) = Rec(nt-1,:)  (solver_acoustic_SS.f90:588)
+    for (auto& s : c->shots) {
+        const size_t n = s.rx.size();
+        if (!n) continue;
+        float* base = c->traces.at<float>(s.o_rec);
+        HIPCHK(hipMemcpyAsync(base + (size_t)(nt - 1) * n, base + (size_t)(nt - 2) * n, n * sizeof(float),
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->have_synth = true;
+    if (store) c->have_snaps = true;
+    return 0;
+}
+
+// Adjoint sweep k = 2..nt with the imaging condition (the per-shot kernels
+// accumulate; the caller zeroes them).  With checkpoints, each earlier
+// segment is recomputed from its checkpoint just before the adjoint reaches
+// the steps that read it.
+int adjoint_sweep(fwi_ctx* c) {
+    const fwi_config& cf = c->cfg;
+    const int nt = cf.nt, ss = cf.store_snap;
+    CHK(attach_storage(c));
+    Arena& ast = c->seg > 0 ? c->astate : c->state;
+    HIPCHK(hipMemsetAsync(ast.base, 0, ast.size, c->stream));
+    hipEvent_t ev0;
+    CHK(timing_begin(c, &ev0));
+    if (c->seg == 0) {
+        CHK(in_graph(c, "adjoint", [&] { return issue_steps(c, c->d_desc_adj, 1, false, 2, nt, 0, 0, 0); }));
+    } else {
+        const int S = c->seg, nseg = num_segments(c, S);
+        int k = 2;
+        for (int j = nseg - 1; j >= 0; --j) {
+            const int lo = j * S + 1, hi = std::min((j + 1) * S, c->nt_store - 1);
+            if (j < nseg - 1 || !c->last_segment_resident) {
+                if (j == 0) HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+                CHK(in_graph(c, "recompute_" + std::to_string(j), [&] {
+                    if (j > 0)
+                        HIPCHK(fwi::launch_copy16(c->state.base, c->ckpt.base + (size_t)(j - 1) * c->state.size,
+                                                  c->state.size / 16, c->stream));
+                    CHK(reset_snapmax(c));
+                    return issue_steps(c, c->d_desc, 0, false, j * S * ss + 1, hi * ss, lo, hi, j * S);
+                }));
+                c->last_segment_resident = false;
+            }
+            const int kend = (c->nt_store - lo) * ss;
+            CHK(in_graph(c, "adjoint_" + std::to_string(j),
+                         [&] { return issue_steps(c, c->d_desc_adj, 1, false, k, kend, 0, 0, j * S); }));
+            k = kend + 1;
         }
-        c->have_synth = true;
-        if (store) c->have_snaps = true;
+        if (k <= nt) CHK(in_graph(c, "adjoint_end", [&] { return issue_steps(c, c->d_desc_adj, 1, false, k, nt, 0, 0, 0); }));
     }
+    CHK(timing_end(c, ev0));
     return 0;
 }
 
@@ -676,7 +932,7 @@ int fwi_observed_from_synthetic(fwi_ctx* c) {
 
 int fwi_forward(fwi_ctx* c, int store_snapshots) {
     if (!c || !c->have_model) return fail("fwi_forward: model not set");
-    return run(c, 0, store_snapshots != 0);
+    return forward_sweep(c, store_snapshots != 0);
 }
 
 int fwi_get_synthetic(fwi_ctx* c, float* out) {
@@ -707,7 +963,7 @@ int fwi_misfit(fwi_ctx* c, double* total) {
 int fwi_adjoint(fwi_ctx* c) {
     if (!c || !c->have_snaps) return fail("fwi_adjoint: run fwi_forward with snapshots first");
     HIPCHK(hipMemsetAsync(c->grads.base, 0, c->grads.size, c->stream));
-    return run(c, 1, false);
+    return adjoint_sweep(c);
 }
 
 int fwi_reduce_gradient(fwi_ctx* c, double scale, const int* water_rows) {
@@ -771,6 +1027,53 @@ int fwi_sync(fwi_ctx* c) {
     return 0;
 }
 
+int fwi_set_schedule(fwi_ctx* c, int segment_slots) {
+    if (!c) return fail("null context");
+    if (segment_slots < 0) return fail("segment_slots must be >= 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->release_storage();
+    c->seg = segment_slots;
+    return 0;
+}
+
+int fwi_plan_schedule(fwi_ctx* c, long long budget_bytes, int* segment_slots) {
+    if (!c || !c->have_shots) return fail("fwi_plan_schedule: set shots first");
+    if (budget_bytes < 0) return fail("negative budget");
+    const size_t budget = (size_t)budget_bytes;
+    int best = -1;
+    if (storage_bytes(c, 0) <= budget) {
+        best = 0;
+    } else {
+        size_t best_b = 0;
+        for (int S = 1; S <= std::max(c->nt_store - 1, 1); ++S) {
+            const size_t b = storage_bytes(c, S);
+            if (b <= budget && (best < 0 || b < best_b)) { best = S; best_b = b; }
+        }
+        if (best < 0) {
+            size_t m = storage_bytes(c, 1);
+            for (int S = 2; S <= std::max(c->nt_store - 1, 1); ++S) m = std::min(m, storage_bytes(c, S));
+            return fail("snapshot budget of " + std::to_string(budget) + " bytes is below the smallest two-level "
+                        "checkpoint schedule (" + std::to_string(m) + " bytes)");
+        }
+    }
+    CHK(fwi_set_schedule(c, best));
+    if (segment_slots) *segment_slots = best;
+    return 0;
+}
+
+long long fwi_schedule_bytes(fwi_ctx* c, int segment_slots) {
+    if (!c || !c->have_shots || segment_slots < 0) return -1;
+    return (long long)storage_bytes(c, segment_slots);
+}
+
+int fwi_set_graphs(fwi_ctx* c, int enable) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->drop_graphs();
+    c->use_graphs = enable != 0;
+    return 0;
+}
+
 int fwi_set_timing(fwi_ctx* c, int enable) {
     if (!c) return fail("null context");
     c->timing = enable != 0;
@@ -796,7 +1099,8 @@ int fwi_get_timing(fwi_ctx* c, double* step_kernel_ms, long long* step_launches,
 
 long long fwi_device_bytes(fwi_ctx* c) {
     if (!c) return 0;
-    return (long long)(c->state.size + c->stat.size + c->traces.size + c->snaps.size + c->grads.size + c->misc.size);
+    return (long long)(c->state.size + c->stat.size + c->traces.size + c->snaps.size + c->grads.size + c->misc.size +
+                       c->astate.size + c->ckpt.size + c->stage.size);
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +1124,7 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
     if (snaps && nt_Store < 0) return fail("negative nt_Store");
     fwi_ctx c;
     CHK(ctx_init(&c, &cfg, snaps ? nt_Store : 0));
+    c.use_graphs = false;          // one sweep per call: capture would not pay
     fwi_shot sh{};
     sh.x0 = 0; sh.w = nx;
     // iprop=2 injects at the receivers (solver_acoustic_SS.f90:114-118)
@@ -831,7 +1136,7 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
     const size_t slot_elems = (size_t)ny * s.Ws;
     if (iprop != 2) {
         if (NumSou) CHK(upload_traces(&c, S, 0));
-        CHK(run(&c, 0, iprop == 1));
+        CHK(forward_sweep(&c, iprop == 1));
         if (NumRec) CHK(download_traces(&c, Rec_point, 1));
         if (iprop == 1) {
             HIPCHK(hipStreamSynchronize(c.stream));
@@ -847,6 +1152,7 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
         }
     } else {
         if (NumRec) CHK(upload_traces(&c, Sadj, 2));
+        CHK(attach_storage(&c));
         std::vector<float> slot(slot_elems, 0.0f);
         for (int m = 0; m <= nt_Store; ++m) {
             const float* F = Fields + (size_t)m * nx * ny;
@@ -860,7 +1166,7 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
             for (int j = 0; j < ny; ++j) slot[(size_t)j * s.Ws + i + s.moff] = kernel[(size_t)i * ny + j];
         HIPCHK(hipMemcpy(c.grads.base + s.o_grad, slot.data(), slot_elems * sizeof(float), hipMemcpyHostToDevice));
         c.have_snaps = true;
-        CHK(run(&c, 1, false));
+        CHK(adjoint_sweep(&c));
         HIPCHK(hipStreamSynchronize(c.stream));
         HIPCHK(hipMemcpy(slot.data(), c.grads.base + s.o_grad, slot_elems * sizeof(float), hipMemcpyDeviceToHost));
         for (int i = 0; i < nx; ++i)

@@ -27,6 +27,23 @@ struct GradShot {     // one shot's FP32 imaging kernel, model column m at k[row
     int x0, W, Ws, moff;
 };
 
+struct QuantShot {    // one shot's staged FP32 snapshot and its 16-bit slots
+    const float* stage;           // [ny][Ws]
+    unsigned short* slots;        // slot s at slots + s * stride
+    float* scale;                 // per-slot scale factor
+    const unsigned int* maxbits;  // per-slot max |value| (float bits)
+    size_t stride;                // elements per slot
+};
+
+// Quantises every shot's staging slot into 16-bit slot `slot` (bf16 != 0:
+// bfloat16, else IEEE half) with a power-of-two scale taken from the slot's
+// maximum, and records the scale.
+hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, int bf16, hipStream_t st);
+// Fills n 32-bit words with v (a kernel, so it may sit in a captured graph).
+hipError_t launch_fill_u32(unsigned int* p, size_t n, unsigned int v, hipStream_t st);
+// Copies n16 16-byte words.
+hipError_t launch_copy16(void* dst, const void* src, size_t n16, hipStream_t st);
+
 hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st);
 // Four copies of the model's c2 (copy q shifted right by q columns, row pitch

@@ -1,6 +1,10 @@
 // Auxiliary kernels of the gradient path: padded c2 construction, trace
-// misfit / adjoint source, FP64 shot-summed gradient.
+// misfit / adjoint source, FP64 shot-summed gradient, snapshot quantisation,
+// fills and copies for checkpoints.
 #include "fwi_aux.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 
 namespace fwi {
 namespace {
@@ -125,7 +129,61 @@ __global__ void gradient_reduce_kernel(const GradShot* __restrict__ shots, int n
     }
 }
 
+// Power-of-two scale that maps the slot's largest |value| into [2^14, 2^15):
+// exact to apply and to undo, so the 16-bit rounding is the only error, and
+// the largest value stays below the half-precision limit 65504.
+__device__ __forceinline__ float snap_scale_of(unsigned int bits) {
+    const float mx = __uint_as_float(bits);
+    if (!(mx > 0.0f) || !(mx <= 3.0e38f)) return 1.0f;
+    int e;
+    (void)frexpf(mx, &e);            // mx = f * 2^e, f in [0.5, 1)
+    return ldexpf(1.0f, e - 15);
+}
+
+__global__ void quantize_kernel(const QuantShot* __restrict__ shots, int slot, int bf16) {
+    const QuantShot q = shots[blockIdx.y];
+    const float sc = snap_scale_of(q.maxbits[slot]);
+    const float inv = 1.0f / sc;
+    unsigned short* __restrict__ out = q.slots + (size_t)slot * q.stride;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < q.stride; e += (size_t)gridDim.x * blockDim.x) {
+        const float v = q.stage[e] * inv;
+        out[e] = bf16 ? __bfloat16_as_ushort(__float2bfloat16(v)) : __half_as_ushort(__float2half(v));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) q.scale[slot] = sc;
+}
+
+__global__ void fill_u32_kernel(unsigned int* __restrict__ p, size_t n, unsigned int v) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) p[e] = v;
+}
+
+__global__ void copy16_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
 }  // namespace
+
+hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, int bf16, hipStream_t st) {
+    if (nshots <= 0) return hipSuccess;
+    const unsigned bx = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (max_stride + 1023) / 1024));
+    hipLaunchKernelGGL(quantize_kernel, dim3(bx, nshots), dim3(256), 0, st, shots, slot, bf16);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned int* p, size_t n, unsigned int v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const unsigned b = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(b), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy16(void* dst, const void* src, size_t n16, hipStream_t st) {
+    if (n16 == 0) return hipSuccess;
+    const unsigned b = (unsigned)std::min<size_t>(8192, (n16 + 255) / 256);
+    hipLaunchKernelGGL(copy16_kernel, dim3(b), dim3(256), 0, st, static_cast<float4*>(dst),
+                       static_cast<const float4*>(src), n16);
+    return hipGetLastError();
+}
 
 hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st) {

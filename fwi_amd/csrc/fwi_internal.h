@@ -38,7 +38,9 @@ struct ShotDev {
     int ntr[2];
     float* rec_tr;                // recorded traces, time-major [t][nrec]
     void* snap;                   // snapshot slots [slot][ny][Ws]
-    const float* snap_scale;      // per-slot scale factors (compressed kinds), or null
+    float* snap_scale;            // per-slot scale factors (16-bit kinds)
+    float* stage;                 // FP32 staging slot [ny][Ws] (16-bit kinds)
+    unsigned int* snap_max;       // per-slot max |snapshot| as float bits (16-bit kinds)
     float* grad;                  // per-shot imaging kernel [ny][Ws] (FP32, reference accumulation)
     size_t snap_stride;           // elements per slot
 };
@@ -79,7 +81,8 @@ struct StepParams {
     int Pm;                       // pitch of the shared interior c2 rows
     int k;                        // Fortran time index of the step being taken (2..nt)
     int mode;                     // 0 forward (inject sources, record receivers), 1 adjoint (inject at receivers)
-    int snap_op;                  // 0 none, 1 store snapshot slot, 2 correlate with slot
+    int snap_op;                  // 0 none, 1 store snapshot slot (FP32 kind) or the FP32 staging slot
+                                  // (16-bit kinds, with the slot's max |value|), 2 correlate with slot
     int slot;
     int snap_kind;                // SnapKind
     int record;                   // 1: sample receivers after the update

@@ -104,16 +104,6 @@ __device__ __forceinline__ void st4(GMEM float* p, float4 v) {
     *reinterpret_cast<GMEM f4v*>(p) = w;
 }
 
-__device__ __forceinline__ void snap_store(GMEM void* base, size_t idx, float v, int kind, float inv_scale) {
-    if (kind == SNAP_F32) {
-        static_cast<GMEM float*>(base)[idx] = v;
-    } else if (kind == SNAP_BF16) {
-        static_cast<GMEM unsigned short*>(base)[idx] = __bfloat16_as_ushort(__float2bfloat16(v * inv_scale));
-    } else {
-        static_cast<GMEM unsigned short*>(base)[idx] = __half_as_ushort(__float2half(v * inv_scale));
-    }
-}
-
 __device__ __forceinline__ float snap_load(const GMEM void* base, size_t idx, int kind, float scale) {
     if (kind == SNAP_F32) return static_cast<const GMEM float*>(base)[idx];
     const unsigned short b = static_cast<const GMEM unsigned short*>(base)[idx];
@@ -123,6 +113,22 @@ __device__ __forceinline__ float snap_load(const GMEM void* base, size_t idx, in
 
 __device__ __forceinline__ GMEM void* slot_base(const GMEM ShotDev& s, const StepParams& prm) {
     return gmem(static_cast<char*>(s.snap)) + (size_t)prm.slot * s.snap_stride * (prm.snap_kind == SNAP_F32 ? 4 : 2);
+}
+
+// Where a storing step writes its FP32 snapshot values: the slot itself for
+// the FP32 kind, the shot's staging slot for the 16-bit kinds (quantised
+// after the step with a scale taken from the slot's largest |value|).
+__device__ __forceinline__ GMEM float* store_base(const GMEM ShotDev& s, const StepParams& prm) {
+    if (prm.snap_kind == SNAP_F32) return static_cast<GMEM float*>(slot_base(s, prm));
+    return gmem(s.stage);
+}
+
+// Folds the wavefront's largest stored |value| into the slot maximum (float
+// bits of non-negative values order like unsigned integers).  Called with
+// all 64 lanes active.
+__device__ __forceinline__ void snap_max_commit(const GMEM ShotDev& s, const StepParams& prm, float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if (threadIdx.x == 0) atomicMax(s.snap_max + prm.slot, __float_as_uint(v));
 }
 
 // Hardware hands block b to XCD b % 8; give each XCD a contiguous range of
@@ -199,14 +205,13 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
     const int yband = t.ya < dn ? 0 : 1;
     const int yb0 = yband == 0 ? 0 : NY - dn;                       // first row of the band
 
-    float snap_scale = 1.0f, snap_inv = 1.0f;
+    float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
-    if (prm.snap_op != 0) {
+    GMEM float* store = nullptr;
+    if (prm.snap_op == 1) store = store_base(s, prm);
+    if (prm.snap_op == 2) {
         snap_slot = slot_base(s, prm);
-        if (prm.snap_kind != SNAP_F32 && s.snap_scale) {
-            snap_scale = gmem(s.snap_scale)[prm.slot];
-            snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
-        }
+        if (prm.snap_kind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
     }
 
     L.src[lane] = 0.0f;
@@ -343,7 +348,8 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
                 const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
                 if (prm.snap_op == 1) {
                     const float f = ((pp - pmr) * 0.5f) / prm.dt;
-                    snap_store(snap_slot, mi, f, prm.snap_kind, snap_inv);
+                    store[mi] = f;
+                    vmax = fmaxf(vmax, fabsf(f));
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
@@ -387,6 +393,7 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
         }
         buf ^= 1;
     }
+    if (prm.snap_op == 1 && prm.snap_kind != SNAP_F32) snap_max_commit(s, prm, vmax);
 }
 
 // Value of v held by lane `src` (all 64 lanes active).
@@ -447,14 +454,13 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
     const int yband = t.ya < dn ? 0 : 1;
     const int yb0 = yband == 0 ? 0 : NY - dn;
 
-    float snap_scale = 1.0f, snap_inv = 1.0f;
+    float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
-    if (prm.snap_op != 0) {
+    GMEM float* store = nullptr;
+    if (prm.snap_op == 1) store = store_base(s, prm);
+    if (prm.snap_op == 2) {
         snap_slot = slot_base(s, prm);
-        if (prm.snap_kind != SNAP_F32 && s.snap_scale) {
-            snap_scale = gmem(s.snap_scale)[prm.slot];
-            snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
-        }
+        if (prm.snap_kind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
     }
 
     L.src[lane] = 0.0f;
@@ -587,7 +593,8 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
                 const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
                 if (prm.snap_op == 1) {
                     const float f = ((pp - pmr) * 0.5f) / prm.dt;
-                    snap_store(snap_slot, mi, f, prm.snap_kind, snap_inv);
+                    store[mi] = f;
+                    vmax = fmaxf(vmax, fabsf(f));
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
@@ -633,6 +640,7 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
             rc = rc1;
         }
     }
+    if (prm.snap_op == 1 && prm.snap_kind != SNAP_F32) snap_max_commit(s, prm, vmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,14 +675,13 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
     GMEM float* __restrict__ rec_t = gmem(s.rec_tr + (size_t)(prm.k - 2) * s.nrec);
     const int snap_op = prm.snap_op;
     const int skind = prm.snap_kind;
-    float snap_scale = 1.0f, snap_inv = 1.0f;
+    float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
-    if (snap_op != 0) {
+    GMEM float* store = nullptr;
+    if (snap_op == 1) store = store_base(s, prm);
+    if (snap_op == 2) {
         snap_slot = slot_base(s, prm);
-        if (skind != SNAP_F32 && s.snap_scale) {
-            snap_scale = gmem(s.snap_scale)[prm.slot];
-            snap_inv = snap_scale > 0.0f ? 1.0f / snap_scale : 0.0f;
-        }
+        if (skind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
     }
     const int mcol = xl - dn + s.moff;      // model storage column of this lane's first cell
     const GMEM float* lanes_p = pc + xl;
@@ -768,18 +775,22 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
             const float ppv[4] = {pp.x, pp.y, pp.z, pp.w};
             const float pmv[4] = {pm.x, pm.y, pm.z, pm.w};
             if (snap_op == 1) {
-                if (full && skind == SNAP_F32) {
+                if (full) {
                     float4 f;
                     f.x = ((pp.x - pm.x) * 0.5f) / prm.dt;
                     f.y = ((pp.y - pm.y) * 0.5f) / prm.dt;
                     f.z = ((pp.z - pm.z) * 0.5f) / prm.dt;
                     f.w = ((pp.w - pm.w) * 0.5f) / prm.dt;
-                    st4(static_cast<GMEM float*>(snap_slot) + mi, f);
+                    st4(store + mi, f);
+                    vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (xl + i < t.xe)
-                            snap_store(snap_slot, mi + i, ((ppv[i] - pmv[i]) * 0.5f) / prm.dt, skind, snap_inv);
+                        if (xl + i < t.xe) {
+                            const float f = ((ppv[i] - pmv[i]) * 0.5f) / prm.dt;
+                            store[mi + i] = f;
+                            vmax = fmaxf(vmax, fabsf(f));
+                        }
                 }
             } else {
                 GMEM float* g = grad + mi;
@@ -825,6 +836,7 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
         if (!row(y0, std::integral_constant<int, 6>{})) break;
         if (!row(y0, std::integral_constant<int, 7>{})) break;
     }
+    if (snap_op == 1 && skind != SNAP_F32) snap_max_commit(s, prm, vmax);
 }
 
 // ---------------------------------------------------------------------------

@@ -55,8 +55,11 @@ def lib():
                      "fwi_observed_from_synthetic", "fwi_forward", "fwi_get_synthetic", "fwi_misfit", "fwi_adjoint",
                      "fwi_reduce_gradient", "fwi_gradient", "fwi_get_gradient", "fwi_get_kernels",
                      "fwi_get_adjoint_sources", "fwi_get_shot_misfits", "fwi_sync", "fwi_solver_acoustic",
-                     "fwi_set_timing", "fwi_get_timing"):
+                     "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs"):
             getattr(L, name).restype = C.c_int
+        L.fwi_plan_schedule.argtypes = [_P, C.c_longlong, C.POINTER(C.c_int)]
+        L.fwi_schedule_bytes.argtypes = [_P, C.c_int]
+        L.fwi_schedule_bytes.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -285,3 +288,19 @@ class Context:
 
     def device_bytes(self):
         return int(lib().fwi_device_bytes(self.h))
+
+    def set_schedule(self, segment_slots):
+        """0: keep every snapshot slot; S > 0: two-level checkpointing with S-slot segments."""
+        _check(lib().fwi_set_schedule(self.h, int(segment_slots)), "fwi_set_schedule")
+
+    def plan_schedule(self, budget_bytes):
+        """Applies the smallest schedule that fits budget_bytes; returns its segment length (0: no checkpoints)."""
+        s = C.c_int(0)
+        _check(lib().fwi_plan_schedule(self.h, C.c_longlong(int(budget_bytes)), C.byref(s)), "fwi_plan_schedule")
+        return s.value
+
+    def schedule_bytes(self, segment_slots):
+        return int(lib().fwi_schedule_bytes(self.h, int(segment_slots)))
+
+    def set_graphs(self, enable=True):
+        _check(lib().fwi_set_graphs(self.h, int(enable)), "fwi_set_graphs")

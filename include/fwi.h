@@ -152,6 +152,24 @@ int fwi_get_adjoint_sources(fwi_ctx* ctx, float* out);
 /* Per-shot misfits of the last fwi_misfit call (FP64). */
 int fwi_get_shot_misfits(fwi_ctx* ctx, double* out);
 
+/* Snapshot schedule (no counterpart in the reference, which keeps every
+ * snapshot in the host array Fields).  segment_slots = 0 (default) stores
+ * every snapshot slot on the GPU.  segment_slots = S > 0 selects two-level
+ * checkpointing: the forward sweep keeps the full propagation state every S
+ * snapshot intervals and only the last S slots; before the adjoint reads an
+ * earlier segment, it is recomputed from its checkpoint (one extra forward
+ * sweep in total).  Results are bit-identical to the stored schedule.
+ * fwi_plan_schedule picks the schedule with the smallest footprint that fits
+ * budget_bytes (0 if every slot fits) and applies it; fwi_schedule_bytes
+ * returns the snapshot-storage bytes of a schedule (-1 before set_shots). */
+int fwi_set_schedule(fwi_ctx* ctx, int segment_slots);
+int fwi_plan_schedule(fwi_ctx* ctx, long long budget_bytes, int* segment_slots);
+long long fwi_schedule_bytes(fwi_ctx* ctx, int segment_slots);
+
+/* Time-step sweeps are captured as HIP graphs on first use and replayed
+ * (enable = 1, default); enable = 0 launches every step directly. */
+int fwi_set_graphs(fwi_ctx* ctx, int enable);
+
 /* Step-kernel timing: when enabled, every sweep records HIP events on the
  * context's stream around its step launches.  fwi_get_timing returns the
  * accumulated GPU time of those spans and the number of step launches in
