@@ -138,6 +138,12 @@ struct fwi_ctx {
     double* d_shot_misfit = nullptr;
     double* d_grad = nullptr;
     int* d_water = nullptr;
+    // NLCG state (FWI.f90:146-178,213,315-317): previous gradient and search
+    // direction in FP64, the base model as lambda = rho vp^2 in FP32.
+    double* d_grad_old = nullptr;
+    double* d_search = nullptr;
+    float* d_lam0 = nullptr;
+    double* d_red = nullptr;              // reduction partials (2 * kDotBlocks) + 2 results
     // Snapshot schedule.  seg == 0: every slot 1..nt_store is stored by the
     // forward sweep.  seg == S > 0: two-level checkpointing -- the forward
     // sweep keeps the full propagation state every S snapshot intervals and
@@ -448,6 +454,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     const size_t o_sm = c->misc.add(nshots * sizeof(double));
     const size_t o_grad = c->misc.add((size_t)cf.ny * cf.nx * sizeof(double));
     const size_t o_water = c->misc.add(cf.nx * sizeof(int));
+    const size_t o_gold = c->misc.add((size_t)cf.ny * cf.nx * sizeof(double));
+    const size_t o_search = c->misc.add((size_t)cf.ny * cf.nx * sizeof(double));
+    const size_t o_lam0 = c->misc.add((size_t)cf.ny * cf.nx * sizeof(float));
+    const size_t o_red = c->misc.add((2 * fwi::kDotBlocks + 2) * sizeof(double));
 
     CHK(c->state.alloc());
     CHK(c->stat.alloc());
@@ -470,6 +480,10 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     c->d_shot_misfit = c->misc.at<double>(o_sm);
     c->d_grad = c->misc.at<double>(o_grad);
     c->d_water = c->misc.at<int>(o_water);
+    c->d_grad_old = c->misc.at<double>(o_gold);
+    c->d_search = c->misc.at<double>(o_search);
+    c->d_lam0 = c->misc.at<float>(o_lam0);
+    c->d_red = c->misc.at<double>(o_red);
     c->d_c2s = c->stat.at<float>(o_c2s);
     c->d_rows = c->stat.at<int>(o_rows);
     c->d_ents = c->stat.at<int2>(o_ents);
@@ -899,15 +913,80 @@ int fwi_set_shots(fwi_ctx* c, int nshots, const fwi_shot* shots) {
     return setup_shots(c, nshots, shots);
 }
 
+// c2 of every shot window and of the shared interior copies from d_vp.
+int rebuild_c2(fwi_ctx* c) {
+    const fwi_config& cf = c->cfg;
+    HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, cf.nx, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
+                                c->stream));
+    HIPCHK(fwi::launch_build_c2_shared(c->d_vp, cf.ny, cf.nx, cf.dt, cf.d, c->d_c2s, c->Pm, c->c2s_stride, c->stream));
+    return 0;
+}
+
 int fwi_set_model(fwi_ctx* c, const float* vp) {
     if (!c || !c->have_shots) return fail("fwi_set_model: set shots first");
     const fwi_config& cf = c->cfg;
     HIPCHK(hipMemcpyAsync(c->d_vp, vp, (size_t)cf.ny * cf.nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, cf.nx, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
-                                c->stream));
-    HIPCHK(fwi::launch_build_c2_shared(c->d_vp, cf.ny, cf.nx, cf.dt, cf.d, c->d_c2s, c->Pm, c->c2s_stride, c->stream));
+    CHK(rebuild_c2(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_model = true;
+    return 0;
+}
+
+int fwi_get_model(fwi_ctx* c, float* vp) {
+    if (!c || !c->have_model) return fail("fwi_get_model: no model");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(vp, c->d_vp, (size_t)c->cfg.ny * c->cfg.nx * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fwi_set_base_model(fwi_ctx* c) {
+    if (!c || !c->have_model) return fail("fwi_set_base_model: no model");
+    HIPCHK(fwi::launch_base_model(c->d_vp, c->d_lam0, (size_t)c->cfg.ny * c->cfg.nx, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_nlcg_direction(fwi_ctx* c, int method, int first, double* beta) {
+    if (!c || !c->have_shots) return fail("fwi_nlcg_direction: no shots");
+    const size_t n = (size_t)c->cfg.ny * c->cfg.nx;
+    double b = 0.0;
+    if (method == 1 && !first) {
+        double sums[2];
+        HIPCHK(fwi::launch_pr_sums(c->d_grad, c->d_grad_old, n, c->d_red, c->d_red + 2 * fwi::kDotBlocks, c->stream));
+        HIPCHK(hipMemcpyAsync(sums, c->d_red + 2 * fwi::kDotBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        b = sums[0] / sums[1];
+    }
+    HIPCHK(fwi::launch_search_update(c->d_grad, c->d_grad_old, c->d_search, n, b, b > 0.0 ? 1 : 0, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (beta) *beta = b;
+    return 0;
+}
+
+int fwi_search_scale(fwi_ctx* c, double* max_ratio) {
+    if (!c || !c->have_shots || !max_ratio) return fail("fwi_search_scale: bad arguments");
+    const size_t n = (size_t)c->cfg.ny * c->cfg.nx;
+    HIPCHK(fwi::launch_search_scale(c->d_search, c->d_lam0, n, c->d_red, c->d_red + 2 * fwi::kDotBlocks, c->stream));
+    HIPCHK(hipMemcpyAsync(max_ratio, c->d_red + 2 * fwi::kDotBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_apply_step(fwi_ctx* c, double step, int commit) {
+    if (!c || !c->have_model) return fail("fwi_apply_step: no model");
+    const size_t n = (size_t)c->cfg.ny * c->cfg.nx;
+    HIPCHK(fwi::launch_apply_step(c->d_search, c->d_lam0, c->d_vp, n, (float)step, commit ? 1 : 0, c->stream));
+    CHK(rebuild_c2(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_get_search(fwi_ctx* c, double* out) {
+    if (!c || !c->have_shots) return fail("fwi_get_search: no shots");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_search, (size_t)c->cfg.ny * c->cfg.nx * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1189,6 +1268,28 @@ void solver_acoustic_ss_(const int* rank, const int* iprop, const int* NumSou, c
         std::fprintf(stderr, "solver_acoustic_SS (libfwi_hip): %s\n", g_err.c_str());
         std::abort();
     }
+}
+
+// Beta_calculous.f90:1-25: Polak-Ribiere beta = sum g (g - g_old) / sum g_old^2,
+// summed in FP64 on the GPU and returned in REAL(4).
+void beta_calculous_(const float* Grad, const float* OldGrad, const int* nx, const int* ny, float* Beta) {
+    const size_t n = (size_t)(*nx) * (*ny);
+    float* d = nullptr;
+    double* red = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * std::max<size_t>(n, 1) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&red, (2 * fwi::kDotBlocks + 2) * sizeof(double));
+    if (e == hipSuccess && n) e = hipMemcpy(d, Grad, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(d + n, OldGrad, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = fwi::launch_pr_sums_f32(d, d + n, n, red, red + 2 * fwi::kDotBlocks, nullptr);
+    double sums[2] = {0.0, 0.0};
+    if (e == hipSuccess) e = hipMemcpy(sums, red + 2 * fwi::kDotBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    if (red) (void)hipFree(red);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "Beta_calculous (libfwi_hip): %s\n", hipGetErrorString(e));
+        std::abort();
+    }
+    *Beta = (float)(sums[0] / sums[1]);
 }
 
 void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySou, const int* nxRec, const int* nyRec,

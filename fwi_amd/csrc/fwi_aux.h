@@ -44,6 +44,26 @@ hipError_t launch_fill_u32(unsigned int* p, size_t n, unsigned int v, hipStream_
 // Copies n16 16-byte words.
 hipError_t launch_copy16(void* dst, const void* src, size_t n16, hipStream_t st);
 
+// Polak-Ribiere sums (Beta_calculous.f90:10-22) in FP64 with a fixed
+// summation order: out[0] = sum g*(g - g_old), out[1] = sum g_old*g_old.
+// partial holds 2*nblk doubles.
+constexpr int kDotBlocks = 256;
+hipError_t launch_pr_sums(const double* g, const double* g_old, size_t n, double* partial, double* out, hipStream_t st);
+hipError_t launch_pr_sums_f32(const float* g, const float* g_old, size_t n, double* partial, double* out,
+                              hipStream_t st);
+// search = -g (+ beta*search when use_beta); g_old = g  (FWI.f90:152-168)
+hipError_t launch_search_update(const double* g, double* g_old, double* search, size_t n, double beta, int use_beta,
+                                hipStream_t st);
+// out[0] = max |search / log(lam0)|  (FWI.f90:175)
+hipError_t launch_search_scale(const double* search, const float* lam0, size_t n, double* partial, double* out,
+                               hipStream_t st);
+// lam = exp(search*step)*lam0, vp = sqrt(lam/1000) in FP32 (FWI.f90:213,315-317);
+// lam0 = lam when commit.
+hipError_t launch_apply_step(const double* search, float* lam0, float* vp, size_t n, float step, int commit,
+                             hipStream_t st);
+// lam0 = 1000 * vp^2  (auxiliar.f90:209-212)
+hipError_t launch_base_model(const float* vp, float* lam0, size_t n, hipStream_t st);
+
 hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st);
 // Four copies of the model's c2 (copy q shifted right by q columns, row pitch

@@ -161,7 +161,147 @@ __global__ void copy16_kernel(float4* __restrict__ dst, const float4* __restrict
         dst[e] = src[e];
 }
 
+// Block-level FP64 reductions in a fixed order (256 threads).
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <class T>
+__global__ void pr_partial_kernel(const T* __restrict__ g, const T* __restrict__ go, size_t n,
+                                  double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double num = 0.0, den = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const double a = g[e], b = go[e];
+        num += a * (a - b);
+        den += b * b;
+    }
+    num = block_sum(num, sh);
+    den = block_sum(den, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = num;
+        partial[gridDim.x + blockIdx.x] = den;
+    }
+}
+
+__global__ void pr_final_kernel(const double* __restrict__ partial, int nblk, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double num = 0.0, den = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        num += partial[b];
+        den += partial[nblk + b];
+    }
+    num = block_sum(num, sh);
+    den = block_sum(den, sh);
+    if (threadIdx.x == 0) {
+        out[0] = num;
+        out[1] = den;
+    }
+}
+
+__global__ void search_update_kernel(const double* __restrict__ g, double* __restrict__ go, double* __restrict__ s,
+                                     size_t n, double beta, int use_beta) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        double v = -g[e];
+        if (use_beta) v = v + beta * s[e];
+        s[e] = v;
+        go[e] = g[e];
+    }
+}
+
+__global__ void search_scale_partial_kernel(const double* __restrict__ s, const float* __restrict__ lam0, size_t n,
+                                            double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(s[e] / log((double)lam0[e])));
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ void max_final_kernel(const double* __restrict__ partial, int nblk, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, partial[b]);
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) out[0] = m;
+}
+
+__global__ void apply_step_kernel(const double* __restrict__ s, float* __restrict__ lam0, float* __restrict__ vp,
+                                  size_t n, float step, int commit) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const float lam = expf((float)s[e] * step) * lam0[e];
+        vp[e] = sqrtf(lam / 1000.0f);
+        if (commit) lam0[e] = lam;
+    }
+}
+
+__global__ void base_model_kernel(const float* __restrict__ vp, float* __restrict__ lam0, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        lam0[e] = 1000.0f * (vp[e] * vp[e]);
+}
+
+unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(4096, std::max<size_t>(1, (n + 255) / 256)); }
+
 }  // namespace
+
+hipError_t launch_pr_sums(const double* g, const double* g_old, size_t n, double* partial, double* out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(pr_partial_kernel<double>, dim3(kDotBlocks), dim3(256), 0, st, g, g_old, n, partial);
+    hipLaunchKernelGGL(pr_final_kernel, dim3(1), dim3(256), 0, st, partial, kDotBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pr_sums_f32(const float* g, const float* g_old, size_t n, double* partial, double* out,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(pr_partial_kernel<float>, dim3(kDotBlocks), dim3(256), 0, st, g, g_old, n, partial);
+    hipLaunchKernelGGL(pr_final_kernel, dim3(1), dim3(256), 0, st, partial, kDotBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_search_update(const double* g, double* g_old, double* search, size_t n, double beta, int use_beta,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(search_update_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, g_old, search, n, beta,
+                       use_beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_search_scale(const double* search, const float* lam0, size_t n, double* partial, double* out,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(search_scale_partial_kernel, dim3(kDotBlocks), dim3(256), 0, st, search, lam0, n, partial);
+    hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, st, partial, kDotBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_step(const double* search, float* lam0, float* vp, size_t n, float step, int commit,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(apply_step_kernel, dim3(grid_for(n)), dim3(256), 0, st, search, lam0, vp, n, step, commit);
+    return hipGetLastError();
+}
+
+hipError_t launch_base_model(const float* vp, float* lam0, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(base_model_kernel, dim3(grid_for(n)), dim3(256), 0, st, vp, lam0, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, int bf16, hipStream_t st) {
     if (nshots <= 0) return hipSuccess;

@@ -55,8 +55,12 @@ def lib():
                      "fwi_observed_from_synthetic", "fwi_forward", "fwi_get_synthetic", "fwi_misfit", "fwi_adjoint",
                      "fwi_reduce_gradient", "fwi_gradient", "fwi_get_gradient", "fwi_get_kernels",
                      "fwi_get_adjoint_sources", "fwi_get_shot_misfits", "fwi_sync", "fwi_solver_acoustic",
-                     "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs"):
+                     "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs",
+                     "fwi_set_base_model", "fwi_nlcg_direction", "fwi_search_scale", "fwi_apply_step",
+                     "fwi_get_model", "fwi_get_search"):
             getattr(L, name).restype = C.c_int
+        L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
+        L.beta_calculous_.restype = None
         L.fwi_plan_schedule.argtypes = [_P, C.c_longlong, C.POINTER(C.c_int)]
         L.fwi_schedule_bytes.argtypes = [_P, C.c_int]
         L.fwi_schedule_bytes.restype = C.c_longlong
@@ -146,6 +150,17 @@ def solver_forward_simple(nxSou, nySou, nxRec, nyRec, S, d, dt, ctmp, nt):
     L.solver_forward_simple_(r(len(nxRec)), r(nxSou), r(nySou), _ptr(nxRec), _ptr(nyRec), _ptr(S), f(d), f(dt),
                              r(ny), r(nx), _ptr(_f32f(ctmp)), r(nt), _ptr(rec))
     return rec
+
+
+def beta_calculous(Grad, OldGrad):
+    """Mirror of the reference's Beta_calculous(Grad, OldGrad, nx, ny, Beta) (Beta_calculous.f90:1):
+    Polak-Ribiere beta of two (ny, nx) REAL(4) gradients, summed in FP64 on the GPU."""
+    g, go = _f32f(Grad), _f32f(OldGrad)
+    assert g.shape == go.shape
+    ny, nx = g.shape
+    b = C.c_float(0)
+    lib().beta_calculous_(_ptr(g), _ptr(go), C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), C.byref(b))
+    return b.value
 
 
 class Context:
@@ -301,6 +316,34 @@ class Context:
 
     def schedule_bytes(self, segment_slots):
         return int(lib().fwi_schedule_bytes(self.h, int(segment_slots)))
+
+    # NLCG on the device-resident gradient (FWI.f90:146-178, 213, 315-317)
+    def set_base_model(self):
+        _check(lib().fwi_set_base_model(self.h), "fwi_set_base_model")
+
+    def nlcg_direction(self, method=1, first=False):
+        """Search direction from the last reduced gradient; returns the Polak-Ribiere beta used (0 if none)."""
+        b = C.c_double(0)
+        _check(lib().fwi_nlcg_direction(self.h, int(method), int(first), C.byref(b)), "fwi_nlcg_direction")
+        return b.value
+
+    def search_scale(self):
+        m = C.c_double(0)
+        _check(lib().fwi_search_scale(self.h, C.byref(m)), "fwi_search_scale")
+        return m.value
+
+    def apply_step(self, step, commit=False):
+        _check(lib().fwi_apply_step(self.h, float(step), int(commit)), "fwi_apply_step")
+
+    def get_model(self):
+        v = np.zeros((self.ny, self.nx), np.float32, order="F")
+        _check(lib().fwi_get_model(self.h, _ptr(v)), "fwi_get_model")
+        return v
+
+    def get_search(self):
+        v = np.zeros((self.ny, self.nx), np.float64, order="F")
+        _check(lib().fwi_get_search(self.h, _ptr(v)), "fwi_get_search")
+        return v
 
     def set_graphs(self, enable=True):
         _check(lib().fwi_set_graphs(self.h, int(enable)), "fwi_set_graphs")

@@ -1,0 +1,120 @@
+"""Host side of the reference's inversion iteration on a device-resident Context.
+
+The reference drives NLCG from Fortran (FWI.f90:146-330): gradient, Polak-Ribiere
+search direction (Beta_calculous.f90), a step-doubling line search on the misfit
+and a quadratic fit of the last three misfits (polyfit.f90).  Here the gradient,
+the FP64 dot products, beta, the search direction and the model update stay on
+the GPU (Context.nlcg_direction / search_scale / apply_step); the scalar
+line-search logic and the quadratic fit are this module.
+
+One deliberate difference: the reference re-derives dt and nt from the CFL
+limit of every trial model and regrids its data (FWI.f90:222-224); a Context
+keeps the dt and nt it was created with, so the caller picks a dt that is
+stable for the velocities the inversion can reach.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def polyfit(nut: float, m0: float, m1: float, m2: float, loopind: int, length: int = 2001):
+    """Step length minimising the quadratic through three misfits (polyfit.f90:1-105).
+
+    loopind == 2: misfits at (0, nut/2, nut), minimum searched over nut*[0, 1];
+    loopind > 2:  misfits at (nut/4, nut/2, nut), minimum over nut*[1/4, 1].
+    Abscissae are standardised (mean 0, sample standard deviation 1) before
+    the 3x3 Vandermonde solve (FP64, partial pivoting as LEGS does); the grid
+    of candidate steps is formed in REAL(4) as the reference does, and the
+    last grid point attaining the minimum wins (polyfit.f90:55-59).  Returns
+    (EndStep, expected misfit); (None, None) for loopind < 2, where the
+    reference leaves EndStep unchanged.
+    """
+    f32 = np.float32
+    nut32 = f32(nut)
+    if loopind == 2:
+        vec = np.array([0.0, 0.5 * float(nut32), float(nut32)])
+        j = np.arange(length, dtype=np.float32)
+        grid = (j * nut32) / f32(length - 1)
+    elif loopind > 2:
+        vec = np.array([0.25 * float(nut32), 0.5 * float(nut32), float(nut32)])
+        j = np.arange(length, dtype=np.float32)
+        grid = f32(0.25) * nut32 + (j * (f32(0.75) * nut32)) / f32(length - 1)
+    else:
+        return None, None
+    mean = vec.sum() / 3.0
+    std = (0.5 * ((vec - mean) ** 2).sum()) ** 0.5
+    v = (vec - mean) / std
+    A = np.stack([v ** 0, v ** 1, v ** 2], 1)
+    coef = np.linalg.solve(A, np.array([m0, m1, m2], np.float64))
+    x = (grid.astype(np.float64) - mean) / std
+    poly = x ** 2 * coef[2] + x * coef[1] + coef[0]
+    mn = poly.min()
+    idx = np.nonzero(poly == mn)[0][-1]
+    return float(f32(grid[idx])), float(mn)
+
+
+@dataclass
+class NLCG:
+    """Iterates the reference's NLCG update (FWI.f90:146-330) on a Context whose
+    shots, sources, observed data and model are set.  method 1: NLCG
+    (Polak-Ribiere), 0: steepest descent."""
+    ctx: object
+    water_rows: object = None
+    scale: float = 1.0
+    method: int = 1
+    max_trials: int = 6
+    it: int = 0
+    end_step: float = 0.0
+    history: list = field(default_factory=list)
+
+    def __post_init__(self):
+        self.ctx.set_base_model()
+
+    def _misfit_only(self):
+        # calculate_misfit_gradient with step = 2: forward modelling and misfit
+        self.ctx.forward(store_snapshots=False)
+        return self.ctx.misfit()
+
+    def iterate(self):
+        """One gradient evaluation, search direction and line search; returns a dict of the scalars."""
+        ctx = self.ctx
+        misfit = ctx.gradient(self.scale, self.water_rows)
+        beta = ctx.nlcg_direction(self.method, first=(self.it == 0))
+        nut = 1.0 / 100.0 / ctx.search_scale() if self.end_step == 0.0 else self.end_step   # FWI.f90:172-178
+        m_t0 = m_t1 = m_t = misfit
+        loopind = looptotal = 0
+        while True:                                    # FWI.f90:200-305
+            loopind += 1
+            looptotal += 1
+            ctx.apply_step(nut, commit=False)          # lambda = exp(Search nut) var_orig
+            if loopind >= 2:
+                m_t0, m_t1 = m_t1, m_t
+            m_t = self._misfit_only()
+            if looptotal > self.max_trials:            # FWI.f90:249-266: give up this iteration
+                ctx.apply_step(0.0, commit=False)
+                self.end_step = 0.0
+                rec = dict(it=self.it, misfit=misfit, beta=beta, step=0.0, trials=looptotal, accepted=False)
+                self.history.append(rec)
+                self.it += 1
+                return rec
+            if m_t - misfit >= 0.0 and loopind == 1:   # step too long: halve and restart
+                nut = nut / 2.0
+                loopind = 0
+                continue
+            if m_t - m_t1 <= 0.0:
+                nut = nut * 2.0
+            else:
+                break
+        if loopind == 2:
+            step, expected = polyfit(nut, misfit, m_t1, m_t, loopind)
+        else:
+            step, expected = polyfit(nut, m_t0, m_t1, m_t, loopind)
+        self.end_step = step
+        ctx.apply_step(step, commit=True)              # lambda = exp(Search EndStep) var_orig; var_orig = lambda
+        rec = dict(it=self.it, misfit=misfit, beta=beta, step=step, expected=expected, trials=looptotal,
+                   accepted=True)
+        self.history.append(rec)
+        self.it += 1
+        return rec

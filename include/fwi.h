@@ -61,6 +61,12 @@ void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySo
                             const int* ny, const int* nx, const float* ctmp, const int* nt,
                             float* Rec_point);
 
+/* Replaces Beta_calculous (src/Beta_calculous.f90:1, called at
+ * src/FWI.f90:157): Polak-Ribiere beta = sum Grad (Grad - OldGrad) /
+ * sum OldGrad^2 over (ny, nx) arrays, summed in FP64 on the GPU (the
+ * reference accumulates in REAL(4)) and returned as REAL(4). */
+void beta_calculous_(const float* Grad, const float* OldGrad, const int* nx, const int* ny, float* Beta);
+
 /* Solver options that the reference hard-codes (src/solver_acoustic_SS.f90:90-93). */
 typedef struct fwi_solver_opts {
     int dpml;          /* CPML thickness in cells; reference: 20 (50 in solver_forward_simple) */
@@ -151,6 +157,24 @@ int fwi_get_kernels(fwi_ctx* ctx, float* out);
 int fwi_get_adjoint_sources(fwi_ctx* ctx, float* out);
 /* Per-shot misfits of the last fwi_misfit call (FP64). */
 int fwi_get_shot_misfits(fwi_ctx* ctx, double* out);
+
+/* NLCG on the device-resident gradient (src/FWI.f90:146-178, 213, 315-317).
+ * fwi_set_base_model: lambda0 = rho vp^2 (rho = 1000, src/auxiliar.f90:209-212)
+ *   of the current model (the reference's var_orig).
+ * fwi_nlcg_direction: method 1 (NLCG) with first = 0 forms the Polak-Ribiere
+ *   beta in FP64 and sets search = -grad + beta search when beta > 0, else
+ *   search = -grad (also for method 0 or first = 1); then grad_old = grad.
+ * fwi_search_scale: max |search / log(lambda0)| (the initial step is
+ *   1/100 of its inverse, src/FWI.f90:175).
+ * fwi_apply_step: model from lambda = exp(search step) lambda0, vp =
+ *   sqrt(lambda / rho) in FP32; commit = 1 also sets lambda0 = lambda. */
+int fwi_set_base_model(fwi_ctx* ctx);
+int fwi_nlcg_direction(fwi_ctx* ctx, int method, int first, double* beta);
+int fwi_search_scale(fwi_ctx* ctx, double* max_ratio);
+int fwi_apply_step(fwi_ctx* ctx, double step, int commit);
+/* Copy the current model vp (ny, nx) or the search direction (FP64). */
+int fwi_get_model(fwi_ctx* ctx, float* vp);
+int fwi_get_search(fwi_ctx* ctx, double* search);
 
 /* Snapshot schedule (no counterpart in the reference, which keeps every
  * snapshot in the host array Fields).  segment_slots = 0 (default) stores

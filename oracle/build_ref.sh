@@ -1,6 +1,7 @@
 #!/bin/bash
-# Builds the reference's own Fortran solver path into oracle/_ref/ as shared
-# libraries that the tests load with ctypes.  This is test infrastructure: the
+# Builds the reference's own Fortran solver path (and Beta_calculous, the
+# NLCG beta) into oracle/_ref/ as shared libraries that the tests load with
+# ctypes.  This is test infrastructure: the
 # product never links or loads anything built here.
 #
 # Sources are compiled where they lie under the reference tree (default
@@ -35,7 +36,7 @@ TMP=$(mktemp -d)
 trap 'rm -rf "$TMP"' EXIT
 mkdir -p "$OUT"
 
-FILES="solver_space_derivs solver_PML_coeffs solver_PML_matrices solver_FD_coeffs Misfit_L2"
+FILES="solver_space_derivs solver_PML_coeffs solver_PML_matrices solver_FD_coeffs Misfit_L2 Beta_calculous"
 
 # free_surface=0 variant of the solver (exactly one line may change)
 sed 's/^\([[:space:]]*\)free_surface=1/\1free_surface=0/' \
