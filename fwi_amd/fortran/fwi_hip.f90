@@ -112,5 +112,51 @@ module fwi_hip
        real(c_float), value :: d, dt
        real(c_float), intent(inout) :: kernel(*), fields(*), rec_point(*)
      end function fwi_solver_acoustic
+
+     ! snapshot schedule: 0 = keep every slot, s > 0 = two-level checkpoints
+     integer(c_int) function fwi_set_schedule(ctx, segment_slots) bind(C, name="fwi_set_schedule")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: segment_slots
+     end function fwi_set_schedule
+
+     integer(c_int) function fwi_plan_schedule(ctx, budget_bytes, segment_slots) bind(C, name="fwi_plan_schedule")
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), value :: budget_bytes
+       integer(c_int), intent(out) :: segment_slots
+     end function fwi_plan_schedule
+
+     ! NLCG on the device-resident gradient (FWI.f90:146-178, 213, 315-317)
+     integer(c_int) function fwi_set_base_model(ctx) bind(C, name="fwi_set_base_model")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function fwi_set_base_model
+
+     integer(c_int) function fwi_nlcg_direction(ctx, method, first, beta) bind(C, name="fwi_nlcg_direction")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: method, first
+       real(c_double), intent(out) :: beta
+     end function fwi_nlcg_direction
+
+     integer(c_int) function fwi_search_scale(ctx, max_ratio) bind(C, name="fwi_search_scale")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: max_ratio
+     end function fwi_search_scale
+
+     integer(c_int) function fwi_apply_step(ctx, step, commit) bind(C, name="fwi_apply_step")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: step
+       integer(c_int), value :: commit
+     end function fwi_apply_step
+
+     integer(c_int) function fwi_get_model(ctx, vp) bind(C, name="fwi_get_model")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: vp(*)
+     end function fwi_get_model
   end interface
 end module fwi_hip
