@@ -18,6 +18,7 @@
 
 #include "fwi.h"
 #include "fwi_aux.h"
+#include "fwi_exact.h"
 #include "fwi_internal.h"
 
 namespace {
@@ -166,6 +167,15 @@ struct fwi_ctx {
     bool use_graphs = true;
     std::map<std::string, hipGraphExec_t> graphs;
 
+    // Exact-adjoint validation mode (allocated on first use).
+    Arena exact;
+    bool exact_hist = false;              // the arena holds the p history (gradient) as well
+    fwi::ExactShot* d_exact = nullptr;
+    fwi::ResidShot* d_resid = nullptr;
+    double* d_exact_partial = nullptr;
+    std::vector<fwi::ExactShot> exact_host;
+    size_t exact_max_cells = 0, exact_max_stride = 0;
+
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;   // step-kernel spans not yet read
@@ -181,7 +191,8 @@ struct fwi_ctx {
     }
     void release() {
         release_storage();
-        state.release(); stat.release(); traces.release(); grads.release(); misc.release();
+        state.release(); stat.release(); traces.release(); grads.release(); misc.release(); exact.release();
+        exact_hist = false;
         have_shots = have_model = have_synth = have_snaps = false;
     }
     void release_storage() {
@@ -673,7 +684,7 @@ int in_graph(fwi_ctx* c, const std::string& key, F&& issue) {
 // slot m = nt_store - k/ss, read from buffer slot m - off (slot 0: the
 // zero slot).
 int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int mode, bool record, int k0, int k1, int lo, int hi,
-                int off) {
+                int off, bool hist = false) {
     const fwi_config& cf = c->cfg;
     fwi::StepParams prm;
     std::memset(&prm, 0, sizeof(prm));
@@ -718,6 +729,8 @@ int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int mode, bool record, int
             }
         }
         HIPCHK(fwi::launch_step(prm, c->stream));
+        if (hist)
+            HIPCHK(fwi::launch_hist_store(desc, c->d_exact, (int)c->shots.size(), c->exact_max_stride, k, c->stream));
         if (prm.snap_op == 1 && q16)
             HIPCHK(fwi::launch_quantize(c->d_quant, (int)c->shots.size(), c->max_slot_elems, prm.slot,
                                         cf.snap_kind == fwi::SNAP_BF16, c->stream));
@@ -749,7 +762,7 @@ int timing_end(fwi_ctx* c, hipEvent_t ev0) {
 
 // Forward sweep k = 2..nt: receivers sampled; with store, snapshots (and the
 // checkpoints of the checkpointed schedule).
-int forward_sweep(fwi_ctx* c, bool store) {
+int forward_sweep(fwi_ctx* c, bool store, bool hist = false) {
     const fwi_config& cf = c->cfg;
     const int nt = cf.nt, ss = cf.store_snap;
     if (store) CHK(attach_storage(c));
@@ -758,7 +771,9 @@ int forward_sweep(fwi_ctx* c, bool store) {
     if (ntr) HIPCHK(hipMemsetAsync(c->traces.base + c->shots[0].o_rec, 0, ntr, c->stream));
     hipEvent_t ev0;
     CHK(timing_begin(c, &ev0));
-    if (!store) {
+    if (hist) {
+        CHK(issue_steps(c, c->d_desc, 0, true, 2, nt, 1, 0, 0, true));
+    } else if (!store) {
         CHK(in_graph(c, "forward", [&] { return issue_steps(c, c->d_desc, 0, true, 2, nt, 1, 0, 0); }));
     } else if (c->seg == 0) {
         CHK(in_graph(c, "forward_store", [&] {
@@ -874,6 +889,132 @@ int download_traces(fwi_ctx* c, float* host, int which /*1 rec, 3 adj*/) {
         if (ntr) HIPCHK(hipMemcpy(tm.data(), c->traces.base + o, tm.size() * sizeof(float), hipMemcpyDeviceToHost));
         from_time_major(tm.data(), nt, ntr, host + off);
         off += (size_t)ntr * nt;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Exact-adjoint validation mode.
+
+// Allocates the exact-adjoint buffers (with the p history when hist) and the
+// per-shot receiver/source cell tables.
+int ensure_exact(fwi_ctx* c, bool hist) {
+    if (c->exact.base && (c->exact_hist || !hist)) return 0;
+    c->exact.release();
+    const int ns = (int)c->shots.size(), nt = c->cfg.nt, NY = c->NY, dn = c->dn;
+    const int nblk = 64;
+    struct Off { size_t a[2], t[4], Xx[2], Px[2], Xy[2], Py[2], rte, gc2, rc, sc, sa, h; };
+    std::vector<Off> o(ns);
+    Arena& A = c->exact;
+    c->exact_max_cells = c->exact_max_stride = 0;
+    for (int i = 0; i < ns; ++i) {
+        const Shot& s = c->shots[i];
+        const size_t f = (size_t)NY * s.P * sizeof(float);
+        for (int b = 0; b < 2; ++b) o[i].a[b] = A.add(f);
+        for (int b = 0; b < 4; ++b) o[i].t[b] = A.add(f);
+        for (int b = 0; b < 2; ++b) {
+            o[i].Xx[b] = A.add((size_t)2 * NY * dn * sizeof(float));
+            o[i].Px[b] = A.add((size_t)2 * NY * dn * sizeof(float));
+            o[i].Xy[b] = A.add((size_t)2 * dn * s.P * sizeof(float));
+            o[i].Py[b] = A.add((size_t)2 * dn * s.P * sizeof(float));
+        }
+        o[i].rte = A.add(f);
+        o[i].gc2 = A.add((size_t)NY * s.P * sizeof(double));
+        o[i].sa = A.add(std::max<size_t>(1, (size_t)nt * s.sx.size()) * sizeof(float));
+        o[i].rc = A.add(std::max<size_t>(1, s.rx.size()) * sizeof(int));
+        o[i].sc = A.add(std::max<size_t>(1, s.sx.size()) * sizeof(int));
+        c->exact_max_cells = std::max(c->exact_max_cells, (size_t)NY * s.NX);
+        c->exact_max_stride = std::max(c->exact_max_stride, (size_t)NY * s.P);
+    }
+    const size_t o_desc = A.add(ns * sizeof(fwi::ExactShot));
+    const size_t o_resid = A.add(ns * sizeof(fwi::ResidShot));
+    const size_t o_part = A.add((size_t)ns * nblk * sizeof(double));
+    if (hist)
+        for (int i = 0; i < ns; ++i) o[i].h = A.add((size_t)(nt + 1) * NY * c->shots[i].P * sizeof(float));
+    CHK(A.alloc());
+    HIPCHK(hipMemset(A.base, 0, A.size));
+    c->exact_hist = hist;
+    c->d_exact = A.at<fwi::ExactShot>(o_desc);
+    c->d_resid = A.at<fwi::ResidShot>(o_resid);
+    c->d_exact_partial = A.at<double>(o_part);
+    c->exact_host.assign(ns, fwi::ExactShot{});
+    std::vector<fwi::ResidShot> rs(ns);
+    for (int i = 0; i < ns; ++i) {
+        const Shot& s = c->shots[i];
+        fwi::ExactShot& x = c->exact_host[i];
+        std::memset(&x, 0, sizeof(x));
+        x.NX = s.NX;
+        x.P = s.P;
+        x.c2 = c->desc[i].c2;
+        for (int b = 0; b < 2; ++b) {
+            x.a[b] = A.at<float>(o[i].a[b]);
+            x.Xx[b] = A.at<float>(o[i].Xx[b]); x.Px[b] = A.at<float>(o[i].Px[b]);
+            x.Xy[b] = A.at<float>(o[i].Xy[b]); x.Py[b] = A.at<float>(o[i].Py[b]);
+        }
+        for (int b = 0; b < 4; ++b) x.t[b] = A.at<float>(o[i].t[b]);
+        x.rte = A.at<float>(o[i].rte);
+        x.gc2 = A.at<double>(o[i].gc2);
+        x.hist_stride = (size_t)NY * s.P;
+        if (hist) { x.hist_w = A.at<float>(o[i].h); x.hist = x.hist_w; }
+        x.nrec = (int)s.rx.size();
+        x.nsrc = (int)s.sx.size();
+        x.e = c->traces.at<float>(s.o_adj);
+        x.srcadj = A.at<float>(o[i].sa);
+        std::vector<int> rc(std::max<size_t>(1, s.rx.size())), sc(std::max<size_t>(1, s.sx.size()), -1);
+        for (size_t r = 0; r < s.rx.size(); ++r) rc[r] = (dn + s.ry[r] - 1) * s.P + (dn + s.rx[r] - 1);
+        // injected sources: model cells only, the last listed source of a cell wins
+        std::map<int, int> last;
+        for (size_t j = 0; j < s.sx.size(); ++j)
+            if (s.sx[j] >= 1 && s.sx[j] <= s.W && s.sy[j] >= 1 && s.sy[j] <= c->cfg.ny)
+                last[(dn + s.sy[j] - 1) * s.P + (dn + s.sx[j] - 1)] = (int)j;
+        for (auto& kv : last) sc[kv.second] = kv.first;
+        HIPCHK(hipMemcpy(A.at<int>(o[i].rc), rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(A.at<int>(o[i].sc), sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
+        x.rec_cell = A.at<int>(o[i].rc);
+        x.src_cell = A.at<int>(o[i].sc);
+        rs[i] = fwi::ResidShot{c->traces.at<float>(s.o_rec), c->traces.at<float>(s.o_obs),
+                               c->traces.at<float>(s.o_adj), (size_t)nt * s.rx.size()};
+    }
+    HIPCHK(hipMemcpy(c->d_exact, c->exact_host.data(), ns * sizeof(fwi::ExactShot), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_resid, rs.data(), ns * sizeof(fwi::ResidShot), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Backward sweep k = nt..2 of the exact adjoint, with e already in the
+// adjoint-trace buffers.  The adjoint fields, intermediates, receiver terms,
+// dJ/dc2 accumulators and source adjoints are zeroed first.
+int exact_backward(fwi_ctx* c) {
+    const int ns = (int)c->shots.size();
+    // working state: everything of the arena except the tables and descriptors
+    for (int i = 0; i < ns; ++i) {
+        const fwi::ExactShot& x = c->exact_host[i];
+        const size_t f = x.hist_stride * sizeof(float);
+        for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(x.a[b], 0, f, c->stream));
+        for (int b = 0; b < 4; ++b) HIPCHK(hipMemsetAsync(x.t[b], 0, f, c->stream));
+        const size_t bx = (size_t)2 * c->NY * c->dn * sizeof(float), by = (size_t)2 * c->dn * x.P * sizeof(float);
+        for (int b = 0; b < 2; ++b) {
+            HIPCHK(hipMemsetAsync(x.Xx[b], 0, bx, c->stream)); HIPCHK(hipMemsetAsync(x.Px[b], 0, bx, c->stream));
+            HIPCHK(hipMemsetAsync(x.Xy[b], 0, by, c->stream)); HIPCHK(hipMemsetAsync(x.Py[b], 0, by, c->stream));
+        }
+        HIPCHK(hipMemsetAsync(x.rte, 0, f, c->stream));
+        HIPCHK(hipMemsetAsync(x.gc2, 0, x.hist_stride * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(x.srcadj, 0, std::max<size_t>(1, (size_t)c->cfg.nt * x.nsrc) * sizeof(float),
+                              c->stream));
+    }
+    fwi::ExactParams q;
+    std::memset(&q, 0, sizeof(q));
+    q.shots = c->d_exact;
+    q.NY = c->NY;
+    q.dn = c->dn;
+    q.fs = c->cfg.free_surface;
+    q.nt = c->cfg.nt;
+    std::memcpy(q.fA, c->fA, sizeof(q.fA));
+    std::memcpy(q.fB, c->fB, sizeof(q.fB));
+    q.cur = 0;
+    for (int k = c->cfg.nt; k >= 2; --k) {
+        q.k = k;
+        HIPCHK(fwi::launch_exact_step(q, ns, c->exact_max_cells, c->stream));
+        q.cur ^= 1;
     }
     return 0;
 }
@@ -1103,6 +1244,68 @@ int fwi_get_shot_misfits(fwi_ctx* c, double* out) {
 int fwi_sync(fwi_ctx* c) {
     if (!c) return fail("null context");
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fwi_exact_transpose(fwi_ctx* c, const float* data, float* src_adjoint) {
+    if (!c || !c->have_model) return fail("fwi_exact_transpose: model not set");
+    CHK(ensure_exact(c, false));
+    CHK(upload_traces(c, data, 2));
+    // the transpose alone needs no history: detach it for this sweep
+    std::vector<fwi::ExactShot> xs = c->exact_host;
+    for (auto& x : xs) x.hist = nullptr;
+    HIPCHK(hipMemcpy(c->d_exact, xs.data(), xs.size() * sizeof(fwi::ExactShot), hipMemcpyHostToDevice));
+    const int r = exact_backward(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_exact, c->exact_host.data(), xs.size() * sizeof(fwi::ExactShot), hipMemcpyHostToDevice));
+    CHK(r);
+    const int nt = c->cfg.nt;
+    size_t off = 0;
+    for (size_t i = 0; i < c->shots.size(); ++i) {
+        const int n = (int)c->shots[i].sx.size();
+        std::vector<float> tm((size_t)nt * std::max(n, 1));
+        if (n) HIPCHK(hipMemcpy(tm.data(), c->exact_host[i].srcadj, tm.size() * sizeof(float), hipMemcpyDeviceToHost));
+        from_time_major(tm.data(), nt, n, src_adjoint + off);
+        off += (size_t)n * nt;
+    }
+    return 0;
+}
+
+int fwi_exact_gradient(fwi_ctx* c, double* misfit, double* grad_vp) {
+    if (!c || !c->have_model) return fail("fwi_exact_gradient: model not set");
+    CHK(ensure_exact(c, true));
+    const fwi_config& cf = c->cfg;
+    const int ns = (int)c->shots.size(), nblk = 64;
+    CHK(forward_sweep(c, false, true));
+    HIPCHK(fwi::launch_residual_l2(c->d_resid, ns, cf.dt, c->d_exact_partial, nblk, c->stream));
+    std::vector<double> part((size_t)ns * nblk);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_exact_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    CHK(exact_backward(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double m = 0.0;
+    for (double v : part) m += v;
+    if (misfit) *misfit = 0.5 * (double)cf.dt * m;
+    if (grad_vp) {
+        const int ny = cf.ny, nx = cf.nx, dn = c->dn;
+        std::vector<float> vp((size_t)ny * nx);
+        HIPCHK(hipMemcpy(vp.data(), c->d_vp, vp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        std::fill(grad_vp, grad_vp + (size_t)ny * nx, 0.0);
+        const double dt = cf.dt, d = cf.d;
+        for (int i = 0; i < ns; ++i) {
+            const Shot& s = c->shots[i];
+            std::vector<double> g((size_t)c->NY * s.P);
+            HIPCHK(hipMemcpy(g.data(), c->exact_host[i].gc2, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+            // fold the padded cells onto the model cells their c2 copies, chain c2 = vp^2 dt^2 / d^2
+            for (int y = 0; y < c->NY; ++y)
+                for (int x = 0; x < s.NX; ++x) {
+                    const int jm = std::min(std::max(y - dn, 0), ny - 1);
+                    const int im = s.x0 + std::min(std::max(x - dn, 0), s.W - 1);
+                    const double v = vp[(size_t)im * ny + jm];
+                    grad_vp[(size_t)im * ny + jm] += g[(size_t)y * s.P + x] * (2.0 * v * dt * dt) / (d * d);
+                }
+        }
+    }
     return 0;
 }
 

@@ -57,7 +57,7 @@ def lib():
                      "fwi_get_adjoint_sources", "fwi_get_shot_misfits", "fwi_sync", "fwi_solver_acoustic",
                      "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs",
                      "fwi_set_base_model", "fwi_nlcg_direction", "fwi_search_scale", "fwi_apply_step",
-                     "fwi_get_model", "fwi_get_search"):
+                     "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient"):
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
@@ -344,6 +344,22 @@ class Context:
         v = np.zeros((self.ny, self.nx), np.float64, order="F")
         _check(lib().fwi_get_search(self.h, _ptr(v)), "fwi_get_search")
         return v
+
+    # exact discrete adjoint (validation mode)
+    def exact_transpose(self, data_list):
+        """Transpose of the source-to-receiver operator: per-shot (nt, nrec) data -> per-shot (nt, nsrc)."""
+        n = sum(s["nsrc"] for s in self.shots) * self.nt
+        out = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_exact_transpose(self.h, _ptr(self._concat(data_list, "nrec")), _ptr(out)),
+               "fwi_exact_transpose")
+        return self._split(out, "nsrc", self.nt)
+
+    def exact_gradient(self):
+        """(misfit dt/2 sum (rec-obs)^2, d misfit / d vp (ny, nx) float64) with the exact discrete adjoint."""
+        m = C.c_double(0)
+        g = np.zeros((self.ny, self.nx), np.float64, order="F")
+        _check(lib().fwi_exact_gradient(self.h, C.byref(m), _ptr(g)), "fwi_exact_gradient")
+        return m.value, g
 
     def set_graphs(self, enable=True):
         _check(lib().fwi_set_graphs(self.h, int(enable)), "fwi_set_graphs")

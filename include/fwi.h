@@ -176,6 +176,19 @@ int fwi_apply_step(fwi_ctx* ctx, double step, int commit);
 int fwi_get_model(fwi_ctx* ctx, float* vp);
 int fwi_get_search(fwi_ctx* ctx, double* search);
 
+/* Exact discrete adjoint (validation mode; no counterpart in the reference,
+ * whose adjoint re-runs the forward propagator on time-reversed residuals).
+ * fwi_exact_transpose applies the transpose of the source-to-receiver
+ * modelling operator: data per shot (nt, nrec_s) in, dJ/d(source sample) per
+ * shot (nt, nsrc_s) out (sources inside the model; a source overridden on a
+ * shared cell gets zeros).  fwi_exact_gradient returns the unnormalised L2
+ * misfit dt/2 sum (rec - obs)^2 (FP64) and its exact gradient with respect to
+ * the model velocity, (ny, nx) column-major FP64, summed over shots.  Both
+ * keep per-shot adjoint state on the device; the gradient also keeps the
+ * full pressure history (nt+1 padded fields per shot). */
+int fwi_exact_transpose(fwi_ctx* ctx, const float* data, float* src_adjoint);
+int fwi_exact_gradient(fwi_ctx* ctx, double* misfit, double* grad_vp);
+
 /* Snapshot schedule (no counterpart in the reference, which keeps every
  * snapshot in the host array Fields).  segment_slots = 0 (default) stores
  * every snapshot slot on the GPU.  segment_slots = S > 0 selects two-level

@@ -407,6 +407,345 @@ void misfit_l2(const T* synth, const T* obs, int nrec, int nt, T dt, T* misfit, 
     *misfit = m;
 }
 
+// ---------------------------------------------------------------------------
+// Exact discrete adjoint (no counterpart in the reference, whose adjoint
+// re-runs the forward propagator on time-reversed residuals).
+//
+// ExactGrid holds the padded grid of one shot, cell-wise: c2 and the CPML
+// factor table indexed by the 1-based distance m = max(dx, dy) into the
+// frame (m = 0 inside), the same per-cell formulation as the GPU kernel, so
+// exact_forward<float> reproduces solver<float> bit for bit.
+//
+// The adjoint is the transpose of one leapfrog step
+//   p_k = 2 p_{k-1} - p_{k-2} + c2 (D2x p_{k-1} + D2y p_{k-1} + X + Y + s_k)
+//   X = Dbx phi_{k-1} + chi_k,  phi_k = A phi_{k-1} + B D1x p_{k-1},
+//   chi_k = A chi_{k-1} + B (D2x p_{k-1} + Dbx phi_{k-1})          (x band; y alike)
+// where D2x, D1x are the clamped derivatives (outer 3 columns copy column 3
+// / NX-4) and Dbx the band-local first derivative (clamped to band columns
+// 3..dn-4).  A clamped operator is S C (stencil S on the computed lines, C
+// copying the clamp line), so its transpose is C^T S^T: fold the outer lines
+// onto the clamp line, then apply the mirrored stencil.  Carried backwards:
+//   a_k  (adjoint of p_k),  Xt_k = c2 a_k + A Xt_{k+1} (of chi_k),
+//   Pt_k = A Pt_{k+1} + Dbx^T u_{k+1} (of phi_k),  u = g + B Xt, g = c2 a,
+//   a_k = e_k + 2 a_{k+1} - a_{k+2} + D2x^T t1x + D1x^T t3x + (y alike),
+//   t1x = g + [band] B Xt,  t3x = [band] B Pt    (all at level k+1).
+// The gradient with respect to c2 is sum_k a_k (p_k - 2 p_{k-1} + p_{k-2})/c2,
+// folded onto the model cells the padded c2 copies (nearest-edge extension)
+// and chained through c2 = vp^2 dt^2 / d^2.
+
+template <class T> struct ExactGrid {
+    int ny, nx, dn, NY, NX, fs;
+    std::vector<T> c2;            // [y][x]
+    std::vector<T> fA, fB;        // [0..dn] by 1-based distance, 0 inside
+    ExactGrid(const Opts& o, int ny_, int nx_, const T* vp, T d, T dt)
+        : ny(ny_), nx(nx_), dn(o.dpml + 6), NY(ny_ + 2 * (o.dpml + 6)), NX(nx_ + 2 * (o.dpml + 6)),
+          fs(o.free_surface), c2((size_t)NY * NX), fA(dn + 1, T(0)), fB(dn + 1, T(0)) {
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const int jm = std::min(std::max(y - dn, 0), ny - 1), im = std::min(std::max(x - dn, 0), nx - 1);
+                const T c = vp[(size_t)im * ny + jm];
+                c2[(size_t)y * NX + x] = ((c * c) * (dt * dt)) / (d * d);
+            }
+        std::vector<T> a(dn), b(dn), m8((size_t)dn * dn * 8);
+        T* M = m8.data();
+        const size_t q = (size_t)dn * dn;
+        pml_coeffs<T>(3, o.dpml, dt, d, dn, a.data(), b.data(), M, M + q, M + 2 * q, M + 3 * q, M + 4 * q,
+                      M + 5 * q, M + 6 * q, M + 7 * q);
+        for (int m = 1; m <= dn; ++m) { fA[m] = a[m - 1]; fB[m] = b[m - 1]; }
+    }
+    int dxv(int x) const { return x < dn ? dn - x : (x >= NX - dn ? x - (NX - dn) + 1 : 0); }
+    int dyv(int y) const {
+        if (y < dn) return fs ? 0 : dn - y;
+        return y >= NY - dn ? y - (NY - dn) + 1 : 0;
+    }
+    bool upd(int y) const { return !fs || y >= dn; }
+    size_t at(int y, int x) const { return (size_t)y * NX + x; }
+};
+
+template <class T> struct Stencil {
+    T c1[4], c2[4];
+    Stencil() {
+        c1[0] = T(0); c1[1] = T(3) / T(4); c1[2] = T(-3) / T(20); c1[3] = T(1) / T(60);
+        c2[0] = (T(-49) / T(18)) / T(2); c2[1] = T(3) / T(2); c2[2] = T(-3) / T(20); c2[3] = T(1) / T(90);
+    }
+    // second difference from values v[-3..3] (v points at the centre), as dd2
+    T d2(const T* v, ptrdiff_t s) const {
+        T r = c2[0] * (v[0] + v[0]);
+        r = r + c2[1] * (v[s] + v[-s]);
+        r = r + c2[2] * (v[2 * s] + v[-2 * s]);
+        r = r + c2[3] * (v[3 * s] + v[-3 * s]);
+        return r;
+    }
+    T d1(const T* v, ptrdiff_t s) const {
+        T r = (v[s] - v[-s]) * c1[1];
+        r = r + (v[2 * s] - v[-2 * s]) * c1[2];
+        r = r + (v[3 * s] - v[-3 * s]) * c1[3];
+        return r;
+    }
+    // weights of the operators as linear maps: out(j) = sum_m w[m+3] in(j+m)
+    T w2(int m) const { return m == 0 ? c2[0] * T(2) : c2[std::abs(m)]; }
+    T w1(int m) const { return m == 0 ? T(0) : (m > 0 ? c1[m] : -c1[-m]); }
+};
+
+// Injection and receiver cells in padded coordinates (model cells only for
+// injection, last listed source wins on a shared cell).
+struct ExactAcq {
+    std::vector<int> src_cell, src_of_cell_tr;   // per source: padded cell or -1; trace injected there
+    std::vector<int> rec_cell;
+};
+
+template <class T>
+ExactAcq exact_acq(const ExactGrid<T>& g, int ns, const int* nxs, const int* nys, int nr, const int* nxr,
+                   const int* nyr) {
+    ExactAcq a;
+    a.src_cell.assign(ns, -1);
+    a.src_of_cell_tr.assign(ns, -1);
+    std::vector<int> last((size_t)g.NY * g.NX, -1);
+    for (int j = 0; j < ns; ++j)
+        if (nxs[j] >= 1 && nxs[j] <= g.nx && nys[j] >= 1 && nys[j] <= g.ny) {
+            const int c = (int)g.at(g.dn + nys[j] - 1, g.dn + nxs[j] - 1);
+            a.src_cell[j] = c;
+            last[c] = j;
+        }
+    for (int j = 0; j < ns; ++j)
+        if (a.src_cell[j] >= 0) a.src_of_cell_tr[j] = last[a.src_cell[j]] == j ? j : -1;
+    for (int r = 0; r < nr; ++r) a.rec_cell.push_back((int)g.at(g.dn + nyr[r] - 1, g.dn + nxr[r] - 1));
+    return a;
+}
+
+// Forward modelling, cell-wise.  rec(nt, nr) column-major as Rec_point;
+// hist (optional) receives p at every time level 0..nt, [k][y][x].
+template <class T>
+void exact_forward(const ExactGrid<T>& g, const ExactAcq& acq, int nt, const T* S, T* rec, T* hist) {
+    const Stencil<T> st;
+    const int NY = g.NY, NX = g.NX, dn = g.dn;
+    const size_t n = (size_t)NY * NX;
+    const int ns = (int)acq.src_cell.size(), nr = (int)acq.rec_cell.size();
+    std::vector<T> p0(n, T(0)), p1(n, T(0)), pn(n, T(0));
+    std::vector<T> phx[2] = {std::vector<T>((size_t)2 * NY * dn, T(0)), std::vector<T>((size_t)2 * NY * dn, T(0))};
+    std::vector<T> chx((size_t)2 * NY * dn, T(0));
+    std::vector<T> phy[2] = {std::vector<T>((size_t)2 * dn * NX, T(0)), std::vector<T>((size_t)2 * dn * NX, T(0))};
+    std::vector<T> chy((size_t)2 * dn * NX, T(0));
+    std::vector<T> src(n, T(0));
+    std::vector<char> inj(n, 0);
+    for (int j = 0; j < ns; ++j)
+        if (acq.src_of_cell_tr[j] >= 0) inj[acq.src_cell[j]] = 1;
+    std::fill(rec, rec + (size_t)nt * nr, T(0));
+    if (hist) std::fill(hist, hist + 2 * n, T(0));
+    for (int k = 2; k <= nt; ++k) {
+        const int o = k & 1;
+        for (int j = 0; j < ns; ++j)
+            if (acq.src_of_cell_tr[j] >= 0) src[acq.src_cell[j]] = S[(size_t)j * nt + (k - 2)];
+        for (int y = 0; y < NY; ++y) {
+            if (!g.upd(y)) continue;
+            const int yc = std::min(std::max(y, 3), NY - 4);
+            const int dyv = g.dyv(y), yband = y < dn ? 0 : 1, rb = y < dn ? y : y - (NY - dn);
+            for (int x = 0; x < NX; ++x) {
+                const size_t i = g.at(y, x);
+                const int xc = std::min(std::max(x, 3), NX - 4);
+                const T d2x = st.d2(&p1[g.at(y, xc)], 1);
+                const T d2y = st.d2(&p1[g.at(yc, x)], NX);
+                const T base = (-p0[i]) + T(2) * p1[i];
+                const T lap = d2x + d2y;
+                const int dxv = g.dxv(x);
+                const T c2 = g.c2[i];
+                T pp;
+                if (dxv == 0 && dyv == 0) {
+                    T qq = lap;
+                    if (inj[i]) qq = qq + src[i];
+                    pp = base + qq * c2;
+                } else {
+                    const int m = std::max(dxv, dyv);
+                    const T A = g.fA[m], B = g.fB[m];
+                    T propx = T(0), propy = T(0);
+                    if (dxv > 0) {
+                        const int band = x < dn ? 0 : 1, b = x < dn ? x : x - (NX - dn);
+                        const int bc = std::min(std::max(b, 3), dn - 4);
+                        const size_t row = ((size_t)band * NY + y) * dn;
+                        const T d1x = st.d1(&p1[g.at(y, xc)], 1);
+                        const T dph = st.d1(&phx[o][row + bc], 1);
+                        const T phn = A * phx[o][row + b] + B * d1x;
+                        const T chn = A * chx[row + b] + B * (d2x + dph);
+                        phx[o ^ 1][row + b] = phn;
+                        chx[row + b] = chn;
+                        propx = dph + chn;
+                    }
+                    if (dyv > 0) {
+                        const int rc = std::min(std::max(rb, 3), dn - 4);
+                        const size_t col = (size_t)yband * dn * NX + x;
+                        const T d1y = st.d1(&p1[g.at(yc, x)], NX);
+                        const T dph = st.d1(&phy[o][col + (size_t)rc * NX], NX);
+                        const T phn = A * phy[o][col + (size_t)rb * NX] + B * d1y;
+                        const T chn = A * chy[col + (size_t)rb * NX] + B * (d2y + dph);
+                        phy[o ^ 1][col + (size_t)rb * NX] = phn;
+                        chy[col + (size_t)rb * NX] = chn;
+                        propy = dph + chn;
+                    }
+                    if (dxv > 0 && dyv > 0) pp = (base + c2 * (lap + propx)) + c2 * propy;
+                    else if (dxv > 0) pp = base + c2 * (lap + propx);
+                    else pp = base + c2 * (lap + propy);
+                }
+                pn[i] = pp;
+            }
+        }
+        for (int r = 0; r < nr; ++r) rec[(size_t)r * nt + (k - 2)] = pn[acq.rec_cell[r]];
+        if (hist) std::copy(pn.begin(), pn.end(), hist + (size_t)k * n);
+        std::swap(p0, p1);
+        std::swap(p1, pn);
+    }
+    for (int r = 0; r < nr; ++r) rec[(size_t)r * nt + (nt - 1)] = rec[(size_t)r * nt + (nt - 2)];
+}
+
+// Exact adjoint.  e(nt, nr): the derivative of the objective with respect to
+// each recorded sample.  srcadj(nt, ns) (optional) receives the derivative
+// with respect to each source sample; grad_vp(ny, nx) (optional, needs hist)
+// receives the derivative with respect to the model velocity.
+template <class T>
+void exact_adjoint(const ExactGrid<T>& g, const ExactAcq& acq, int nt, const T* e, const T* hist, T d, T dt,
+                   const T* vp, T* srcadj, double* grad_vp) {
+    const Stencil<T> st;
+    const int NY = g.NY, NX = g.NX, dn = g.dn;
+    const size_t n = (size_t)NY * NX;
+    const int ns = (int)acq.src_cell.size(), nr = (int)acq.rec_cell.size();
+    std::vector<T> a2(n, T(0)), a1(n, T(0)), an(n, T(0)), rte(n, T(0));
+    std::vector<T> t1x(n), t1y(n), t3x(n), t3y(n);
+    std::vector<T> Xx[2], Px[2], Xy[2], Py[2];
+    for (int b = 0; b < 2; ++b) {
+        Xx[b].assign((size_t)2 * NY * dn, T(0)); Px[b].assign((size_t)2 * NY * dn, T(0));
+        Xy[b].assign((size_t)2 * dn * NX, T(0)); Py[b].assign((size_t)2 * dn * NX, T(0));
+    }
+    std::vector<double> gc2(n, 0.0);
+    if (srcadj) std::fill(srcadj, srcadj + (size_t)nt * ns, T(0));
+    auto xband = [&](int x, int& band, int& b) { band = x < dn ? 0 : 1; b = x < dn ? x : x - (NX - dn); };
+    auto yband = [&](int y, int& band, int& rb) { band = y < dn ? 0 : 1; rb = y < dn ? y : y - (NY - dn); };
+    int cur = 0;   // Xx[cur] etc. hold level k+1
+    for (int k = nt; k >= 2; --k) {
+        // level k+1 intermediates (zero at k = nt)
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const size_t i = g.at(y, x);
+                const T gk = g.c2[i] * a1[i];
+                t1x[i] = gk; t1y[i] = gk; t3x[i] = T(0); t3y[i] = T(0);
+                if (!g.upd(y)) continue;
+                const int dxv = g.dxv(x), dyv = g.dyv(y), m = std::max(dxv, dyv);
+                const T B = g.fB[m];
+                if (dxv > 0) {
+                    int band, b; xband(x, band, b);
+                    const size_t q = ((size_t)band * NY + y) * dn + b;
+                    t1x[i] = gk + B * Xx[cur][q];
+                    t3x[i] = B * Px[cur][q];
+                }
+                if (dyv > 0) {
+                    int band, rb; yband(y, band, rb);
+                    const size_t q = (size_t)band * dn * NX + (size_t)rb * NX + x;
+                    t1y[i] = gk + B * Xy[cur][q];
+                    t3y[i] = B * Py[cur][q];
+                }
+            }
+        // receiver term
+        std::fill(rte.begin(), rte.end(), T(0));
+        for (int r = 0; r < nr; ++r) {
+            T v = e[(size_t)r * nt + (k - 2)];
+            if (k == nt) v = v + e[(size_t)r * nt + (nt - 1)];
+            rte[acq.rec_cell[r]] += v;
+        }
+        // folded line value of a clamped operator's input at computed line j
+        auto foldx = [&](const std::vector<T>& t, int y, int j) -> T {
+            if (j < 3 || j > NX - 4) return T(0);
+            T v = t[g.at(y, j)];
+            if (j == 3) v = v + t[g.at(y, 0)] + t[g.at(y, 1)] + t[g.at(y, 2)];
+            if (j == NX - 4) v = v + t[g.at(y, NX - 3)] + t[g.at(y, NX - 2)] + t[g.at(y, NX - 1)];
+            return v;
+        };
+        auto foldy = [&](const std::vector<T>& t, int j, int x) -> T {
+            if (j < 3 || j > NY - 4) return T(0);
+            T v = t[g.at(j, x)];
+            if (j == 3) v = v + t[g.at(0, x)] + t[g.at(1, x)] + t[g.at(2, x)];
+            if (j == NY - 4) v = v + t[g.at(NY - 3, x)] + t[g.at(NY - 2, x)] + t[g.at(NY - 1, x)];
+            return v;
+        };
+        for (int y = 0; y < NY; ++y) {
+            if (!g.upd(y)) continue;
+            for (int x = 0; x < NX; ++x) {
+                const size_t i = g.at(y, x);
+                T v = rte[i] + T(2) * a1[i] - a2[i];
+                for (int m = -3; m <= 3; ++m) {
+                    v = v + st.w2(m) * foldx(t1x, y, x - m) + st.w1(m) * foldx(t3x, y, x - m);
+                    v = v + st.w2(m) * foldy(t1y, y - m, x) + st.w1(m) * foldy(t3y, y - m, x);
+                }
+                an[i] = v;
+            }
+        }
+        // memory-variable adjoints at level k
+        const int nx_ = cur ^ 1;
+        for (int y = 0; y < NY; ++y) {
+            if (!g.upd(y)) continue;
+            for (int x = 0; x < NX; ++x) {
+                const int dxv = g.dxv(x), dyv = g.dyv(y), m = std::max(dxv, dyv);
+                const T A = g.fA[m];
+                const size_t i = g.at(y, x);
+                if (dxv > 0) {
+                    int band, b; xband(x, band, b);
+                    const size_t row = ((size_t)band * NY + y) * dn;
+                    // Dbx^T u: u = t1x on band cells of this row, folded onto band columns 3..dn-4
+                    auto U = [&](int j) -> T {
+                        if (j < 3 || j > dn - 4) return T(0);
+                        auto ub = [&](int bb) { return t1x[g.at(y, band == 0 ? bb : bb + (NX - dn))]; };
+                        T v = ub(j);
+                        if (j == 3) v = v + ub(0) + ub(1) + ub(2);
+                        if (j == dn - 4) v = v + ub(dn - 3) + ub(dn - 2) + ub(dn - 1);
+                        return v;
+                    };
+                    T dtu = T(0);
+                    for (int mm = -3; mm <= 3; ++mm) dtu = dtu + st.w1(mm) * U(b - mm);
+                    Px[nx_][row + b] = A * Px[cur][row + b] + dtu;
+                    Xx[nx_][row + b] = g.c2[i] * an[i] + A * Xx[cur][row + b];
+                }
+                if (dyv > 0) {
+                    int band, rb; yband(y, band, rb);
+                    const size_t col = (size_t)band * dn * NX + x;
+                    auto U = [&](int j) -> T {
+                        if (j < 3 || j > dn - 4) return T(0);
+                        auto ub = [&](int bb) { return t1y[g.at(band == 0 ? bb : bb + (NY - dn), x)]; };
+                        T v = ub(j);
+                        if (j == 3) v = v + ub(0) + ub(1) + ub(2);
+                        if (j == dn - 4) v = v + ub(dn - 3) + ub(dn - 2) + ub(dn - 1);
+                        return v;
+                    };
+                    T dtu = T(0);
+                    for (int mm = -3; mm <= 3; ++mm) dtu = dtu + st.w1(mm) * U(rb - mm);
+                    Py[nx_][col + (size_t)rb * NX] = A * Py[cur][col + (size_t)rb * NX] + dtu;
+                    Xy[nx_][col + (size_t)rb * NX] = g.c2[i] * an[i] + A * Xy[cur][col + (size_t)rb * NX];
+                }
+            }
+        }
+        cur = nx_;
+        if (grad_vp && hist) {
+            const T* hk = hist + (size_t)k * n;
+            const T* h1 = hist + (size_t)(k - 1) * n;
+            const T* h2 = hist + (size_t)(k - 2) * n;
+            for (size_t i = 0; i < n; ++i)
+                gc2[i] += (double)an[i] * (((double)hk[i] - 2.0 * (double)h1[i] + (double)h2[i]) / (double)g.c2[i]);
+        }
+        if (srcadj)
+            for (int j = 0; j < ns; ++j)
+                if (acq.src_of_cell_tr[j] >= 0)
+                    srcadj[(size_t)j * nt + (k - 2)] = g.c2[acq.src_cell[j]] * an[acq.src_cell[j]];
+        std::swap(a2, a1);
+        std::swap(a1, an);
+    }
+    if (grad_vp) {
+        std::fill(grad_vp, grad_vp + (size_t)g.ny * g.nx, 0.0);
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const int jm = std::min(std::max(y - dn, 0), g.ny - 1), im = std::min(std::max(x - dn, 0), g.nx - 1);
+                const double v = vp[(size_t)im * g.ny + jm];
+                grad_vp[(size_t)im * g.ny + jm] += gc2[g.at(y, x)] * (2.0 * v * (double)dt * (double)dt) /
+                                                  ((double)d * (double)d);
+            }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -439,6 +778,31 @@ struct oracle_opts { int dpml; int free_surface; };
 
 ORACLE_SOLVER(float, f32)
 ORACLE_SOLVER(double, f64)
+
+// Exact discrete adjoint pair (see exact_forward / exact_adjoint).
+#define ORACLE_EXACT(T, SUF)                                                                                \
+    int oracle_exact_forward_##SUF(const oracle_opts* o, int ns, const int* nxs, const int* nys, int nr,   \
+                                   const int* nxr, const int* nyr, const T* S, T d, T dt, int ny, int nx,  \
+                                   const T* vp, int nt, T* rec, T* hist) {                                  \
+        Opts op{o->dpml, o->free_surface};                                                                  \
+        if (op.dpml < 2 || nt < 2) return -1;                                                               \
+        ExactGrid<T> g(op, ny, nx, vp, d, dt);                                                              \
+        exact_forward<T>(g, exact_acq<T>(g, ns, nxs, nys, nr, nxr, nyr), nt, S, rec, hist);                \
+        return 0;                                                                                           \
+    }                                                                                                       \
+    int oracle_exact_adjoint_##SUF(const oracle_opts* o, int ns, const int* nxs, const int* nys, int nr,   \
+                                   const int* nxr, const int* nyr, T d, T dt, int ny, int nx, const T* vp, \
+                                   int nt, const T* e, const T* hist, T* srcadj, double* grad_vp) {         \
+        Opts op{o->dpml, o->free_surface};                                                                  \
+        if (op.dpml < 2 || nt < 2) return -1;                                                               \
+        ExactGrid<T> g(op, ny, nx, vp, d, dt);                                                              \
+        exact_adjoint<T>(g, exact_acq<T>(g, ns, nxs, nys, nr, nxr, nyr), nt, e, hist, d, dt, vp, srcadj,   \
+                         grad_vp);                                                                          \
+        return 0;                                                                                           \
+    }
+
+ORACLE_EXACT(float, f32)
+ORACLE_EXACT(double, f64)
 
 // solver_forward_simple.f90:4-542: the same scheme with dPML=50, one source
 // injected at one cell, free surface on, no snapshots.

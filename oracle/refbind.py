@@ -181,3 +181,49 @@ def oracle_misfit(synth, obs, dt, precision="f32"):
     getattr(lib, f"oracle_misfit_l2_{precision}")(_ptr(synth), _ptr(obs), C.c_int(nrec), C.c_int(nt), ct(dt),
                                                  C.byref(mis), _ptr(adj))
     return float(mis.value), adj
+
+
+def oracle_exact_forward(precision, nxsou, nysou, nxrec, nyrec, S, d, dt, vp, nt, dpml=20, free_surface=0,
+                         history=False):
+    """Cell-wise forward of the oracle's exact-adjoint pair.  Returns (rec (nt, nrec), hist or None),
+    hist[k] = p at time level k on the padded grid, (nt+1, NY, NX)."""
+    lib = oracle()
+    dt_np = np.float64 if precision == "f64" else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    ny, nx = vp.shape
+    dn = dpml + 6
+    nxsou, nysou, nxrec, nyrec = (np.ascontiguousarray(a, dtype=np.int32) for a in (nxsou, nysou, nxrec, nyrec))
+    S = _f(S, dt_np)
+    vp = _f(vp, dt_np)
+    rec = np.zeros((nt, len(nxrec)), dtype=dt_np, order="F")
+    hist = np.zeros((nt + 1, ny + 2 * dn, nx + 2 * dn), dtype=dt_np) if history else None
+    fn = getattr(lib, f"oracle_exact_forward_{precision}")
+    fn.restype = C.c_int
+    st = fn(C.byref(OracleOpts(dpml, free_surface)), C.c_int(len(nxsou)), _ptr(nxsou), _ptr(nysou),
+            C.c_int(len(nxrec)), _ptr(nxrec), _ptr(nyrec), _ptr(S), ct(d), ct(dt), C.c_int(ny), C.c_int(nx),
+            _ptr(vp), C.c_int(nt), _ptr(rec), _ptr(hist) if hist is not None else None)
+    assert st == 0
+    return rec, hist
+
+
+def oracle_exact_adjoint(precision, nxsou, nysou, nxrec, nyrec, e, d, dt, vp, nt, hist=None, dpml=20,
+                         free_surface=0):
+    """Exact transpose of oracle_exact_forward applied to e (nt, nrec).  Returns (srcadj (nt, nsrc),
+    grad_vp (ny, nx) float64 or None when hist is None)."""
+    lib = oracle()
+    dt_np = np.float64 if precision == "f64" else np.float32
+    ct = C.c_double if dt_np == np.float64 else C.c_float
+    ny, nx = vp.shape
+    nxsou, nysou, nxrec, nyrec = (np.ascontiguousarray(a, dtype=np.int32) for a in (nxsou, nysou, nxrec, nyrec))
+    e = _f(e, dt_np)
+    vp = _f(vp, dt_np)
+    srcadj = np.zeros((nt, len(nxsou)), dtype=dt_np, order="F")
+    grad = np.zeros((ny, nx), dtype=np.float64, order="F") if hist is not None else None
+    fn = getattr(lib, f"oracle_exact_adjoint_{precision}")
+    fn.restype = C.c_int
+    st = fn(C.byref(OracleOpts(dpml, free_surface)), C.c_int(len(nxsou)), _ptr(nxsou), _ptr(nysou),
+            C.c_int(len(nxrec)), _ptr(nxrec), _ptr(nyrec), ct(d), ct(dt), C.c_int(ny), C.c_int(nx), _ptr(vp),
+            C.c_int(nt), _ptr(e), _ptr(np.ascontiguousarray(hist, dtype=dt_np)) if hist is not None else None,
+            _ptr(srcadj), _ptr(grad) if grad is not None else None)
+    assert st == 0
+    return srcadj, grad
