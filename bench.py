@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=5000)
-    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0, help="rows per work unit (0: autotune)")
+    ap.add_argument("--tune-steps", type=int, default=60, help="forward steps timed per autotuner candidate")
     ap.add_argument("--snap", choices=["f32", "bf16", "f16"], default="f32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
@@ -162,6 +163,12 @@ def main():
                           rows_per_task=a.rows)
     t_setup = time.perf_counter()
     ctx.set_shots(acq.shots)
+    tuned = None
+    if not a.rows:
+        rows, us = ctx.autotune(a.tune_steps)
+        tuned = {"rows_per_task": rows, "step_us": us, "steps_timed": a.tune_steps}
+        print(f"[bench] autotune: {rows} rows per work unit ({us:.1f} us per forward step)", file=sys.stderr,
+              flush=True)
     ctx.set_sources(acq.wavelets)
     ctx.set_model(acq.vp_true)
     ctx.forward(store_snapshots=False)           # observed data from the true model
@@ -216,7 +223,8 @@ def main():
             "nshots": len(acq.shots),
             "nt": acq.nt,
             "snapshots": a.snap,
-            "rows_per_task": a.rows or 8,
+            "rows_per_task": ctx.rows_per_task(),
+            "autotuned": tuned,
         },
         "seconds_per_gradient": wall / a.steps,
         "misfit": misfit,

@@ -1309,6 +1309,53 @@ int fwi_exact_gradient(fwi_ctx* c, double* misfit, double* grad_vp) {
     return 0;
 }
 
+int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
+    if (!c || !c->have_shots) return fail("fwi_autotune: set shots first");
+    if (steps < 1) return fail("fwi_autotune: steps must be >= 1");
+    // shot definitions survive the re-setups below
+    const std::vector<Shot> saved = c->shots;
+    std::vector<fwi_shot> defs(saved.size());
+    for (size_t i = 0; i < saved.size(); ++i) {
+        const Shot& s = saved[i];
+        defs[i] = fwi_shot{s.x0, s.W, (int)s.sx.size(), s.sx.data(), s.sy.data(), (int)s.rx.size(), s.rx.data(),
+                           s.ry.data()};
+    }
+    const int k1 = std::min(c->cfg.nt, steps + 1);
+    const int candidates[] = {4, 6, 8, 12, 16, 24, 32};
+    int best = -1;
+    double best_us = 0.0;
+    std::vector<float> vp((size_t)c->cfg.ny * c->cfg.nx, 2000.0f);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int r : candidates) {
+        if (r > fwi::MAX_TASK_ROWS) continue;
+        c->rows = r;
+        CHK(setup_shots(c, (int)defs.size(), defs.data()));
+        CHK(fwi_set_model(c, vp.data()));
+        HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+        CHK(issue_steps(c, c->d_desc, 0, false, 2, std::min(k1, 4), 1, 0, 0));      // warm-up
+        HIPCHK(hipEventRecord(e0, c->stream));
+        CHK(issue_steps(c, c->d_desc, 0, false, 2, k1, 1, 0, 0));
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1000.0 * ms / (k1 - 1);
+        if (best < 0 || us < best_us) { best = r; best_us = us; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->rows = best;
+    c->cfg.rows_per_task = best;
+    CHK(setup_shots(c, (int)defs.size(), defs.data()));
+    if (best_rows) *best_rows = best;
+    if (best_step_us) *best_step_us = best_us;
+    return 0;
+}
+
+int fwi_get_rows_per_task(fwi_ctx* c) { return c ? c->rows : -1; }
+
 int fwi_set_schedule(fwi_ctx* c, int segment_slots) {
     if (!c) return fail("null context");
     if (segment_slots < 0) return fail("segment_slots must be >= 0");

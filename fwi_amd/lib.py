@@ -57,7 +57,8 @@ def lib():
                      "fwi_get_adjoint_sources", "fwi_get_shot_misfits", "fwi_sync", "fwi_solver_acoustic",
                      "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs",
                      "fwi_set_base_model", "fwi_nlcg_direction", "fwi_search_scale", "fwi_apply_step",
-                     "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient"):
+                     "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient",
+                     "fwi_autotune", "fwi_get_rows_per_task"):
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
@@ -344,6 +345,16 @@ class Context:
         v = np.zeros((self.ny, self.nx), np.float64, order="F")
         _check(lib().fwi_get_search(self.h, _ptr(v)), "fwi_get_search")
         return v
+
+    def autotune(self, steps=100):
+        """Pick rows per work unit by timing forward steps; call right after set_shots.
+        Returns (rows, microseconds per step)."""
+        r, us = C.c_int(0), C.c_double(0)
+        _check(lib().fwi_autotune(self.h, int(steps), C.byref(r), C.byref(us)), "fwi_autotune")
+        return r.value, us.value
+
+    def rows_per_task(self):
+        return int(lib().fwi_get_rows_per_task(self.h))
 
     # exact discrete adjoint (validation mode)
     def exact_transpose(self, data_list):

@@ -203,6 +203,16 @@ int fwi_set_schedule(fwi_ctx* ctx, int segment_slots);
 int fwi_plan_schedule(fwi_ctx* ctx, long long budget_bytes, int* segment_slots);
 long long fwi_schedule_bytes(fwi_ctx* ctx, int segment_slots);
 
+/* Autotuner (no counterpart in the reference): times `steps` forward steps of
+ * the context's own shots (all shots in every launch) for each candidate
+ * number of rows per work unit, keeps the fastest and rebuilds the task
+ * list.  Call it right after fwi_set_shots: it re-creates the shot state,
+ * so the model, sources and data are uploaded afterwards.  Returns the rows
+ * chosen and the measured time per step. */
+int fwi_autotune(fwi_ctx* ctx, int steps, int* best_rows, double* best_step_us);
+/* Rows per work unit of the current task list. */
+int fwi_get_rows_per_task(fwi_ctx* ctx);
+
 /* Time-step sweeps are captured as HIP graphs on first use and replayed
  * (enable = 1, default); enable = 0 launches every step directly. */
 int fwi_set_graphs(fwi_ctx* ctx, int enable);

@@ -1,0 +1,36 @@
+"""The autotuner picks a rows-per-work-unit value from its candidates by timing
+forward steps of the context's own shots; the task decomposition never changes
+a cell's arithmetic, so the autotuned context's gradient is bit-identical to a
+context with a fixed decomposition."""
+import numpy as np
+import pytest
+
+import fwi_amd
+from fwi_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(full, shots, wav, rows, tune):
+    ctx = fwi_amd.Context(full.ny, full.nx, np.float32(full.h), np.float32(full.dt), full.nt, 10, dpml=20,
+                          free_surface=0, rows_per_task=rows)
+    ctx.set_shots(shots)
+    info = ctx.autotune(20) if tune else None
+    ctx.set_sources(wav)
+    ctx.set_model(full.vp_true)
+    ctx.forward(False)
+    ctx.observed_from_synthetic()
+    ctx.set_model(full.vp_init)
+    m = ctx.gradient(1.0, full.water_rows)
+    return info, ctx.rows_per_task(), m, ctx.get_gradient()
+
+
+def test_autotuned_gradient_is_bit_identical():
+    full = synth.marine_64shot_2000x400(seed=0, nshots=64, nt=400)
+    shots, wav = full.shots[:6], full.wavelets[:6]
+    info, rows, m1, g1 = _run(full, shots, wav, 0, True)
+    print(f"\nautotune: {info[0]} rows per work unit, {info[1]:.1f} us per forward step (6 shots)")
+    assert info[0] == rows and rows in (4, 6, 8, 12, 16, 24, 32) and info[1] > 0
+    _, _, m0, g0 = _run(full, shots, wav, 7, False)
+    assert m1 == m0
+    assert np.abs(g0).max() > 0 and np.array_equal(g1, g0)
