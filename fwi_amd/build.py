@@ -36,6 +36,11 @@ HIP_FLAGS = [
 ]
 
 
+# hipFFT for the batched trace filter (trace conditioning, not the stencil path)
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+LINK = [f"-L{ROCM}/lib", "-lhipfft", f"-Wl,-rpath,{ROCM}/lib"]
+
+
 def hipcc():
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -47,7 +52,7 @@ def build_hip(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-           *srcs, "-o", tmp]
+           *srcs, *LINK, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

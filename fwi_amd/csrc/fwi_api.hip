@@ -20,6 +20,7 @@
 #include "fwi_aux.h"
 #include "fwi_exact.h"
 #include "fwi_internal.h"
+#include "fwi_trace.h"
 
 namespace {
 
@@ -878,14 +879,14 @@ int upload_traces(fwi_ctx* c, const float* host, int which /*0 src,1 obs,2 adj*/
     return 0;
 }
 
-int download_traces(fwi_ctx* c, float* host, int which /*1 rec, 3 adj*/) {
+int download_traces(fwi_ctx* c, float* host, int which /*0 src, 1 rec, 2 obs, 3 adj*/) {
     const int nt = c->cfg.nt;
     HIPCHK(hipStreamSynchronize(c->stream));
     size_t off = 0;
     for (auto& s : c->shots) {
-        const int ntr = (int)s.rx.size();
+        const int ntr = (int)(which == 0 ? s.sx.size() : s.rx.size());
         std::vector<float> tm((size_t)ntr * nt);
-        const size_t o = which == 1 ? s.o_rec : s.o_adj;
+        const size_t o = which == 0 ? s.o_src : (which == 1 ? s.o_rec : (which == 2 ? s.o_obs : s.o_adj));
         if (ntr) HIPCHK(hipMemcpy(tm.data(), c->traces.base + o, tm.size() * sizeof(float), hipMemcpyDeviceToHost));
         from_time_major(tm.data(), nt, ntr, host + off);
         off += (size_t)ntr * nt;
@@ -1229,6 +1230,16 @@ int fwi_get_kernels(fwi_ctx* c, float* out) {
     return 0;
 }
 
+int fwi_get_sources(fwi_ctx* c, float* out) {
+    if (!c || !c->have_shots) return fail("fwi_get_sources: no shots");
+    return download_traces(c, out, 0);
+}
+
+int fwi_get_observed(fwi_ctx* c, float* out) {
+    if (!c || !c->have_shots) return fail("fwi_get_observed: no shots");
+    return download_traces(c, out, 2);
+}
+
 int fwi_get_adjoint_sources(fwi_ctx* c, float* out) {
     if (!c) return fail("null context");
     return download_traces(c, out, 3);
@@ -1355,6 +1366,88 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
 }
 
 int fwi_get_rows_per_task(fwi_ctx* c) { return c ? c->rows : -1; }
+
+// ---------------------------------------------------------------------------
+// Trace conditioning (time_filter.f90, interpolation_akima.f90)
+
+int fwi_time_filter(float* data, int nt, float dt, int ntr, int typef, float f1, float fc) {
+    if (nt < 1 || ntr < 0 || !(dt > 0.0f)) return fail("fwi_time_filter: bad arguments");
+    if (typef < 1 || typef > 3) return fail("fwi_time_filter: typef must be 1, 2 or 3");
+    if (ntr == 0) return 0;
+    float* d = nullptr;
+    const size_t bytes = (size_t)nt * ntr * sizeof(float);
+    HIPCHK(hipMalloc(&d, bytes));
+    std::string err;
+    int r = hipMemcpy(d, data, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : fail("fwi_time_filter: upload failed");
+    if (!r) r = fwi::trace_filter(d, nt, ntr, dt, typef, f1, fc, nullptr, &err) ? fail(err) : 0;
+    if (!r && hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) r = fail("fwi_time_filter: download failed");
+    (void)hipFree(d);
+    return r;
+}
+
+int fwi_resample_akima(const float* in, float dt_in, int nt_in, float* out, float dt_out, int nt_out, int ntr) {
+    if (nt_in < 2 || nt_out < 1 || ntr < 0) return fail("fwi_resample_akima: bad arguments");
+    if (ntr == 0) return 0;
+    float *di = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc(&di, (size_t)nt_in * ntr * sizeof(float)));
+    int r = 0;
+    if (hipMalloc(&dout, (size_t)nt_out * ntr * sizeof(float)) != hipSuccess) r = fail("fwi_resample_akima: allocation failed");
+    if (!r && hipMemcpy(di, in, (size_t)nt_in * ntr * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        r = fail("fwi_resample_akima: upload failed");
+    if (!r && fwi::akima_resample(di, dt_in, nt_in, dout, dt_out, nt_out, ntr, nullptr) != hipSuccess)
+        r = fail("fwi_resample_akima: kernel launch failed");
+    if (!r && hipMemcpy(out, dout, (size_t)nt_out * ntr * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        r = fail("fwi_resample_akima: download failed");
+    (void)hipFree(di);
+    if (dout) (void)hipFree(dout);
+    return r;
+}
+
+// Raw traces per shot (nt_data, n_s) column-major, shot after shot: filtered
+// and resampled onto the context's (nt, dt) on the device, stored as the
+// sources (which = 0) or the observed data (which = 1).
+int set_raw_traces(fwi_ctx* c, const float* raw, int nt_data, float dt_data, int typef, float f1, float f2,
+                   int which) {
+    if (!c || !c->have_shots) return fail("set raw traces: set shots first");
+    if (nt_data < 2 || !(dt_data > 0.0f)) return fail("set raw traces: bad sampling");
+    if (typef < 0 || typef > 3) return fail("set raw traces: typef must be 0 (no filter), 1, 2 or 3");
+    const int nt = c->cfg.nt;
+    size_t total = 0;
+    for (auto& s : c->shots) total += which == 0 ? s.sx.size() : s.rx.size();
+    if (total == 0) return 0;
+    float *d_raw = nullptr, *d_res = nullptr;
+    HIPCHK(hipMalloc(&d_raw, total * nt_data * sizeof(float)));
+    int r = 0;
+    std::string err;
+    if (hipMalloc(&d_res, total * nt * sizeof(float)) != hipSuccess) r = fail("set raw traces: allocation failed");
+    if (!r && hipMemcpyAsync(d_raw, raw, total * nt_data * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        r = fail("set raw traces: upload failed");
+    if (!r && typef > 0 && fwi::trace_filter(d_raw, nt_data, (int)total, dt_data, typef, f1, f2, c->stream, &err))
+        r = fail(err);
+    if (!r && fwi::akima_resample(d_raw, dt_data, nt_data, d_res, c->cfg.dt, nt, (int)total, c->stream) != hipSuccess)
+        r = fail("set raw traces: resampling failed");
+    size_t off = 0;
+    for (auto& s : c->shots) {
+        if (r) break;
+        const int n = (int)(which == 0 ? s.sx.size() : s.rx.size());
+        float* dst = c->traces.at<float>(which == 0 ? s.o_src : s.o_obs);
+        if (n && fwi::traces_to_time_major(d_res + off * nt, nt, n, dst, c->stream) != hipSuccess)
+            r = fail("set raw traces: transpose failed");
+        off += n;
+    }
+    if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = fail("set raw traces: stream error");
+    (void)hipFree(d_raw);
+    if (d_res) (void)hipFree(d_res);
+    return r;
+}
+
+int fwi_set_sources_raw(fwi_ctx* c, const float* raw, int nt_data, float dt_data, int typef, float f1, float f2) {
+    return set_raw_traces(c, raw, nt_data, dt_data, typef, f1, f2, 0);
+}
+
+int fwi_set_observed_raw(fwi_ctx* c, const float* raw, int nt_data, float dt_data, int typef, float f1, float f2) {
+    return set_raw_traces(c, raw, nt_data, dt_data, typef, f1, f2, 1);
+}
 
 int fwi_set_schedule(fwi_ctx* c, int segment_slots) {
     if (!c) return fail("null context");
@@ -1540,6 +1633,25 @@ void beta_calculous_(const float* Grad, const float* OldGrad, const int* nx, con
         std::abort();
     }
     *Beta = (float)(sums[0] / sums[1]);
+}
+
+// time_filter.f90:1: Data_trace(nt, Num) filtered in place on the GPU.
+void time_filter_(float* Data_trace, const int* nt, const float* dt, const int* Num, const int* typef,
+                  const float* f1, const float* fc) {
+    if (fwi_time_filter(Data_trace, *nt, *dt, *Num, *typef, *f1, *fc) != 0) {
+        std::fprintf(stderr, "time_filter (libfwi_hip): %s\n", g_err.c_str());
+        std::abort();
+    }
+}
+
+// interpolation_akima.f90:1: a LOGICAL function (returns .true.).
+int interpolation_akima_(const float* Data_tmp, const float* dt_Data, const int* nt_Data, float* Data_trace,
+                         const int* NumRec, const float* dt, const int* nt) {
+    if (fwi_resample_akima(Data_tmp, *dt_Data, *nt_Data, Data_trace, *dt, *nt, *NumRec) != 0) {
+        std::fprintf(stderr, "interpolation_akima (libfwi_hip): %s\n", g_err.c_str());
+        std::abort();
+    }
+    return 1;
 }
 
 void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySou, const int* nxRec, const int* nyRec,

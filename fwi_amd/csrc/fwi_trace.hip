@@ -1,0 +1,282 @@
+// Trace conditioning on the GPU: the Butterworth filter of time_filter.f90 /
+// filters.f90 and the Akima resampling of interpolation_akima.f90 /
+// akima433.f90, batched over traces.
+//
+// Filter: each trace is placed at the end of a 24x zero-padded record
+// (time_filter.f90:17-19), transformed with a batched real-to-complex FFT
+// (hipFFT), multiplied by the filter response on bins 0..N/2 and transformed
+// back; the last nt samples, divided by N (FFTPACK's forward transform carries
+// the 1/N), replace the trace.  The reference transforms the real record as a
+// complex sequence and keeps the real part; with a real record and a response
+// that is conjugate-symmetric except at the Nyquist bin, R2C/C2R gives the
+// same real part.  The response follows filters.f90:81-184: order-6 analog
+// Butterworth, H = 1/B(if) with the bin frequency accumulated in REAL(4)
+// (f = f + fe/ts), the DC bin forced to 1 (low-pass) or 0.
+//
+// Akima: ACM Algorithm 433 as modified in akima433.f90 (INTRPL), one thread
+// per output sample, the Fortran's statement order (its per-interval
+// coefficient reuse and EQUIVALENCE'd temporaries do not change values).
+#include "fwi_trace.h"
+
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace fwi {
+
+void butterworth_response(int ts, float dt, int typef, float f1, float fc, std::vector<float2>& H) {
+    const int n = 6;                           // filters.f90:22
+    const float pi = 3.14159265f;
+    const float fe = 1.0f / dt;
+    const float fs = 40.0f;                    // filters.f90:30 (high-pass start frequency)
+    const float f2 = fc;
+    const float f0 = std::sqrt(f1 * f2);
+    const float Q = f0 / (f2 - f1);
+    const int half = ts / 2 + 1;
+    H.assign(half, float2{0.0f, 0.0f});
+    const float step = fe / (float)ts;
+    float f = 0.0f;
+    for (int j = 0; j < half; ++j) {
+        const double w = f;
+        std::complex<double> z;
+        if (typef == 1) z = std::complex<double>(0.0, w / (double)fc);
+        else if (typef == 2) z = (double)fs / std::complex<double>(0.0, w);
+        else z = (double)Q * (std::complex<double>(0.0, w / (double)f0) + (double)f0 / std::complex<double>(0.0, w));
+        std::complex<double> B(1.0, 0.0);
+        for (int k = 1; k <= n / 2; ++k) {
+            const double c = std::cos((double)(((float)(2 * k + n - 1) * pi) / (float)(2 * n)));
+            B = B * (z * z - 2.0 * z * c + 1.0);
+        }
+        const std::complex<double> h = 1.0 / B;
+        H[j] = float2{(float)h.real(), (float)h.imag()};
+        f = f + step;
+    }
+    H[0] = typef == 1 ? float2{1.0f, 0.0f} : float2{0.0f, 0.0f};
+}
+
+namespace {
+
+__global__ void pad_kernel(const float* __restrict__ in, int nt, int N, int ntr, float* __restrict__ rec) {
+    const size_t total = (size_t)ntr * N;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / N), i = (int)(e % N);
+        rec[e] = i >= N - nt ? in[(size_t)t * nt + (i - (N - nt))] : 0.0f;
+    }
+}
+
+__global__ void apply_response_kernel(float2* __restrict__ spec, const float2* __restrict__ H, int half, int ntr) {
+    const size_t total = (size_t)ntr * half;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const float2 a = spec[e], h = H[e % half];
+        spec[e] = make_float2(a.x * h.x - a.y * h.y, a.x * h.y + a.y * h.x);
+    }
+}
+
+__global__ void extract_kernel(const float* __restrict__ rec, int nt, int N, int ntr, float inv_n,
+                               float* __restrict__ out) {
+    const size_t total = (size_t)ntr * nt;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / nt), i = (int)(e % nt);
+        out[e] = rec[(size_t)t * N + (N - nt) + i] * inv_n;
+    }
+}
+
+// One output sample of INTRPL on the grid X(i) = (i-1) dt_in, U(k) = (k-1) dt_out.
+__device__ float akima_point(const float* __restrict__ Yp, int L, float dt_in, float UK) {
+    const int L0 = L, LM1 = L0 - 1, LM2 = LM1 - 1, LP1 = L0 + 1;
+    auto x = [&](int i) { return (float)(i - 1) * dt_in; };
+    auto y = [&](int i) { return Yp[i - 1]; };
+    int I;
+    if (LM2 == 0) I = 2;
+    else if (UK >= x(L0)) I = LP1;
+    else if (UK < x(1)) I = 1;
+    else {
+        int IMN = 2, IMX = L0;
+        do {
+            I = (IMN + IMX) / 2;
+            if (UK >= x(I)) IMN = I + 1;
+            else IMX = I;
+        } while (IMX > IMN);
+        I = IMX;
+    }
+    int J = I;
+    if (J == 1) J = 2;
+    if (J == LP1) J = L0;
+    float X3 = x(J - 1), Y3 = y(J - 1), X4 = x(J), Y4 = y(J);
+    float A3 = X4 - X3;
+    float M3 = (Y4 - Y3) / A3;
+    float M1 = 0, M2 = 0, M4 = 0, M5 = 0, A2 = 0, A4 = 0, X2 = 0, Y2 = 0, X5 = 0, Y5 = 0;
+    if (LM2 == 0) {
+        M2 = M3;
+        M4 = M3;
+    } else {
+        bool to41 = J == 2;
+        if (!to41) {
+            X2 = x(J - 2); Y2 = y(J - 2);
+            A2 = X3 - X2;
+            M2 = (Y3 - Y2) / A2;
+            if (J == L0) M4 = M3 + M3 - M2;
+            else to41 = true;
+        }
+        if (to41) {
+            X5 = x(J + 1); Y5 = y(J + 1);
+            A4 = X5 - X4;
+            M4 = (Y5 - Y4) / A4;
+            if (J == 2) M2 = M3 + M3 - M4;
+        }
+    }
+    if (J <= 3) M1 = M2 + M2 - M3;
+    else {
+        const float A1 = X2 - x(J - 3);
+        M1 = (Y2 - y(J - 3)) / A1;
+    }
+    if (J >= LM1) M5 = M4 + M4 - M3;
+    else {
+        const float A5 = x(J + 2) - X5;
+        M5 = (y(J + 2) - Y5) / A5;
+    }
+    float T3 = 0, T4 = 0;
+    const bool lp1 = I == LP1;
+    if (!lp1) {
+        float W2 = fabsf(M4 - M3), W3 = fabsf(M2 - M1);
+        float SW = W2 + W3;
+        if (!(SW != 0.0f)) { W2 = 0.5f; W3 = 0.5f; SW = 1.0f; }
+        T3 = (W2 * M2 + W3 * M3) / SW;
+    }
+    if (I == 1) {
+        T4 = T3;
+        const float SA = A3 + A4;
+        T3 = 0.5f * (M1 + M2 - A4 * (A3 - A4) * (M3 - M4) / (SA * SA));
+        X3 = X3 - A4;
+        Y3 = Y3 - M2 * A4;
+        A3 = A4;
+        M3 = M2;
+    } else {
+        float W3 = fabsf(M5 - M4), W4 = fabsf(M3 - M2);
+        float SW = W3 + W4;
+        if (!(SW != 0.0f)) { W3 = 0.5f; W4 = 0.5f; SW = 1.0f; }
+        T4 = (W3 * M3 + W4 * M4) / SW;
+        if (lp1) {
+            T3 = T4;
+            const float SA = A2 + A3;
+            T4 = 0.5f * (M4 + M5 - A2 * (A2 - A3) * (M2 - M3) / (SA * SA));
+            X3 = X4;
+            Y3 = Y4;
+            A3 = A2;
+            M3 = M4;
+        }
+    }
+    const float Q2 = (2.0f * (M3 - T3) + M3 - T4) / A3;
+    const float Q3 = (-M3 - M3 + T3 + T4) / (A3 * A3);
+    const float DX = UK - X3;
+    return Y3 + DX * (T3 + DX * (Q2 + DX * Q3));
+}
+
+__global__ void akima_kernel(const float* __restrict__ in, float dt_in, int nt_in, float* __restrict__ out,
+                             float dt_out, int nt_out, int ntr) {
+    const size_t total = (size_t)ntr * nt_out;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / nt_out), k = (int)(e % nt_out);
+        out[e] = akima_point(in + (size_t)t * nt_in, nt_in, dt_in, (float)k * dt_out);
+    }
+}
+
+__global__ void time_major_kernel(const float* __restrict__ in, int nt, int ntr, float* __restrict__ out) {
+    const size_t total = (size_t)ntr * nt;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / ntr), r = (int)(e % ntr);
+        out[e] = in[(size_t)r * nt + t];
+    }
+}
+
+unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>(16384, std::max<size_t>(1, (n + 255) / 256)); }
+
+}  // namespace
+
+hipError_t traces_to_time_major(const float* in, int nt, int ntr, float* out, hipStream_t st) {
+    if (ntr <= 0 || nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(time_major_kernel, dim3(blocks_for((size_t)ntr * nt)), dim3(256), 0, st, in, nt, ntr, out);
+    return hipGetLastError();
+}
+
+int trace_filter(float* traces, int nt, int ntr, float dt, int typef, float f1, float fc, hipStream_t st,
+                 std::string* err) {
+    if (ntr <= 0 || nt <= 0) return 0;
+    const int N = 24 * nt;                     // time_filter.f90:14-15
+    const int half = N / 2 + 1;
+    std::vector<float2> H;
+    butterworth_response(N, dt, typef, f1, fc, H);
+    // chunk of traces sized to ~1 GiB of FFT buffers
+    const size_t per_trace = (size_t)N * sizeof(float) + (size_t)half * sizeof(float2);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)ntr, ((size_t)1 << 30) / per_trace));
+    float* rec = nullptr;
+    float2* spec = nullptr;
+    float2* dH = nullptr;
+    hipfftHandle fwd = 0, bwd = 0;
+    bool have_fwd = false, have_bwd = false;
+    int status = 0;
+    auto fail = [&](const std::string& m) { if (err) *err = m; status = -1; };
+    do {
+        if (hipMalloc(&rec, (size_t)chunk * N * sizeof(float)) != hipSuccess ||
+            hipMalloc(&spec, (size_t)chunk * half * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&dH, (size_t)half * sizeof(float2)) != hipSuccess) {
+            fail("trace filter: device allocation failed");
+            break;
+        }
+        if (hipMemcpyAsync(dH, H.data(), half * sizeof(float2), hipMemcpyHostToDevice, st) != hipSuccess) {
+            fail("trace filter: response upload failed");
+            break;
+        }
+        int n = N;
+        if (hipfftPlanMany(&fwd, 1, &n, nullptr, 1, N, nullptr, 1, half, HIPFFT_R2C, chunk) != HIPFFT_SUCCESS) {
+            fail("trace filter: hipfftPlanMany (R2C) failed");
+            break;
+        }
+        have_fwd = true;
+        if (hipfftPlanMany(&bwd, 1, &n, nullptr, 1, half, nullptr, 1, N, HIPFFT_C2R, chunk) != HIPFFT_SUCCESS) {
+            fail("trace filter: hipfftPlanMany (C2R) failed");
+            break;
+        }
+        have_bwd = true;
+        hipfftSetStream(fwd, st);
+        hipfftSetStream(bwd, st);
+        for (int t0 = 0; t0 < ntr && status == 0; t0 += chunk) {
+            const int nb = std::min(chunk, ntr - t0);
+            float* tr = traces + (size_t)t0 * nt;
+            if (nb < chunk) (void)hipMemsetAsync(rec, 0, (size_t)chunk * N * sizeof(float), st);
+            hipLaunchKernelGGL(pad_kernel, dim3(blocks_for((size_t)nb * N)), dim3(256), 0, st, tr, nt, N, nb, rec);
+            if (hipfftExecR2C(fwd, rec, reinterpret_cast<hipfftComplex*>(spec)) != HIPFFT_SUCCESS) {
+                fail("trace filter: R2C failed");
+                break;
+            }
+            hipLaunchKernelGGL(apply_response_kernel, dim3(blocks_for((size_t)nb * half)), dim3(256), 0, st, spec, dH,
+                               half, nb);
+            if (hipfftExecC2R(bwd, reinterpret_cast<hipfftComplex*>(spec), rec) != HIPFFT_SUCCESS) {
+                fail("trace filter: C2R failed");
+                break;
+            }
+            hipLaunchKernelGGL(extract_kernel, dim3(blocks_for((size_t)nb * nt)), dim3(256), 0, st, rec, nt, N, nb,
+                               1.0f / (float)N, tr);
+            if (hipGetLastError() != hipSuccess) fail("trace filter: kernel launch failed");
+        }
+        if (status == 0 && hipStreamSynchronize(st) != hipSuccess) fail("trace filter: stream error");
+    } while (false);
+    if (have_fwd) hipfftDestroy(fwd);
+    if (have_bwd) hipfftDestroy(bwd);
+    if (rec) (void)hipFree(rec);
+    if (spec) (void)hipFree(spec);
+    if (dH) (void)hipFree(dH);
+    return status;
+}
+
+hipError_t akima_resample(const float* in, float dt_in, int nt_in, float* out, float dt_out, int nt_out, int ntr,
+                          hipStream_t st) {
+    if (ntr <= 0 || nt_out <= 0) return hipSuccess;
+    hipLaunchKernelGGL(akima_kernel, dim3(blocks_for((size_t)ntr * nt_out)), dim3(256), 0, st, in, dt_in, nt_in, out,
+                       dt_out, nt_out, ntr);
+    return hipGetLastError();
+}
+
+}  // namespace fwi

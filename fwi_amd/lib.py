@@ -58,10 +58,17 @@ def lib():
                      "fwi_set_timing", "fwi_get_timing", "fwi_set_schedule", "fwi_plan_schedule", "fwi_set_graphs",
                      "fwi_set_base_model", "fwi_nlcg_direction", "fwi_search_scale", "fwi_apply_step",
                      "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient",
-                     "fwi_autotune", "fwi_get_rows_per_task"):
+                     "fwi_autotune", "fwi_get_rows_per_task", "fwi_get_sources", "fwi_get_observed",
+                     "fwi_time_filter", "fwi_resample_akima", "fwi_set_sources_raw", "fwi_set_observed_raw"):
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
+        L.time_filter_.restype = None
+        L.interpolation_akima_.restype = C.c_int
+        L.fwi_time_filter.argtypes = [_P, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.fwi_resample_akima.argtypes = [_P, C.c_float, C.c_int, _P, C.c_float, C.c_int, C.c_int]
+        for name in ("fwi_set_sources_raw", "fwi_set_observed_raw"):
+            getattr(L, name).argtypes = [_P, _P, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float]
         L.fwi_plan_schedule.argtypes = [_P, C.c_longlong, C.POINTER(C.c_int)]
         L.fwi_schedule_bytes.argtypes = [_P, C.c_int]
         L.fwi_schedule_bytes.restype = C.c_longlong
@@ -151,6 +158,26 @@ def solver_forward_simple(nxSou, nySou, nxRec, nyRec, S, d, dt, ctmp, nt):
     L.solver_forward_simple_(r(len(nxRec)), r(nxSou), r(nySou), _ptr(nxRec), _ptr(nyRec), _ptr(S), f(d), f(dt),
                              r(ny), r(nx), _ptr(_f32f(ctmp)), r(nt), _ptr(rec))
     return rec
+
+
+def time_filter(data, dt, typef, f1, fc):
+    """Mirror of the reference's time_filter(Data_trace(nt, Num), nt, dt, Num, typef, f1, fc)
+    (time_filter.f90:1) on the GPU; returns the filtered copy."""
+    d = _f32f(data).copy(order="F")
+    nt, num = d.shape
+    _check(lib().fwi_time_filter(_ptr(d), nt, float(dt), num, int(typef), float(f1), float(fc)), "fwi_time_filter")
+    return d
+
+
+def interpolation_akima(data, dt_data, dt, nt):
+    """Mirror of interpolation_akima(Data_tmp, dt_Data, nt_Data, Data_trace, NumRec, dt, nt)
+    (interpolation_akima.f90:1) on the GPU; returns Data_trace (nt, NumRec)."""
+    d = _f32f(data)
+    nt_in, ntr = d.shape
+    out = np.zeros((nt, ntr), np.float32, order="F")
+    _check(lib().fwi_resample_akima(_ptr(d), float(dt_data), nt_in, _ptr(out), float(dt), int(nt), ntr),
+           "fwi_resample_akima")
+    return out
 
 
 def beta_calculous(Grad, OldGrad):
@@ -243,6 +270,38 @@ class Context:
         buf = np.zeros(max(n, 1), np.float32)
         _check(lib().fwi_get_synthetic(self.h, _ptr(buf)), "fwi_get_synthetic")
         return self._split(buf, "nrec", self.nt)
+
+    def sources(self):
+        n = sum(s["nsrc"] for s in self.shots) * self.nt
+        buf = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_get_sources(self.h, _ptr(buf)), "fwi_get_sources")
+        return self._split(buf, "nsrc", self.nt)
+
+    def observed(self):
+        n = sum(s["nrec"] for s in self.shots) * self.nt
+        buf = np.zeros(max(n, 1), np.float32)
+        _check(lib().fwi_get_observed(self.h, _ptr(buf)), "fwi_get_observed")
+        return self._split(buf, "nrec", self.nt)
+
+    def _raw(self, per_shot, key, nt_data):
+        parts = []
+        for s, a in zip(self.shots, per_shot):
+            a = _f32f(a)
+            assert a.shape == (nt_data, s[key]), (a.shape, s[key])
+            parts.append(a.ravel(order="F"))
+        return np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1, np.float32))
+
+    def set_sources_raw(self, S_list, dt_data, typef=3, f1=3.0, f2=5.0):
+        """Per-shot (nt_data, nsrc) traces on dt_data: filtered and resampled onto the context's grid."""
+        nt_data = _f32f(S_list[0]).shape[0]
+        _check(lib().fwi_set_sources_raw(self.h, _ptr(self._raw(S_list, "nsrc", nt_data)), nt_data, float(dt_data),
+                                         int(typef), float(f1), float(f2)), "fwi_set_sources_raw")
+
+    def set_observed_raw(self, D_list, dt_data, typef=3, f1=3.0, f2=5.0):
+        """Per-shot (nt_data, nrec) field data on dt_data: filtered and resampled onto the context's grid."""
+        nt_data = _f32f(D_list[0]).shape[0]
+        _check(lib().fwi_set_observed_raw(self.h, _ptr(self._raw(D_list, "nrec", nt_data)), nt_data, float(dt_data),
+                                          int(typef), float(f1), float(f2)), "fwi_set_observed_raw")
 
     def adjoint_sources(self):
         n = sum(s["nrec"] for s in self.shots) * self.nt

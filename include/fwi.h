@@ -61,6 +61,21 @@ void solver_forward_simple_(const int* NumRec, const int* nxSou, const int* nySo
                             const int* ny, const int* nx, const float* ctmp, const int* nt,
                             float* Rec_point);
 
+/* Replaces time_filter (src/time_filter.f90:1, called at
+ * src/calculate_misfit_gradient.f90:193-194): Data_trace(nt, Num) filtered in
+ * place with the order-6 Butterworth of src/filters.f90 (typef 1 low-pass at
+ * fc, 2 high-pass, 3 band-pass f1..fc), 24x zero padding, batched FFTs on the
+ * GPU.  Aborts on error like the reference's runtime. */
+void time_filter_(float* Data_trace, const int* nt, const float* dt, const int* Num, const int* typef,
+                  const float* f1, const float* fc);
+/* Replaces the LOGICAL function interpolation_akima
+ * (src/interpolation_akima.f90:1, called at
+ * src/calculate_misfit_gradient.f90:230,233): Data_tmp(nt_Data, NumRec) on
+ * (i-1)*dt_Data resampled onto (i-1)*dt with ACM 433 Akima interpolation
+ * (src/akima433.f90) on the GPU; returns 1 (.true.). */
+int interpolation_akima_(const float* Data_tmp, const float* dt_Data, const int* nt_Data, float* Data_trace,
+                         const int* NumRec, const float* dt, const int* nt);
+
 /* Replaces Beta_calculous (src/Beta_calculous.f90:1, called at
  * src/FWI.f90:157): Polak-Ribiere beta = sum Grad (Grad - OldGrad) /
  * sum OldGrad^2 over (ny, nx) arrays, summed in FP64 on the GPU (the
@@ -157,6 +172,21 @@ int fwi_get_kernels(fwi_ctx* ctx, float* out);
 int fwi_get_adjoint_sources(fwi_ctx* ctx, float* out);
 /* Per-shot misfits of the last fwi_misfit call (FP64). */
 int fwi_get_shot_misfits(fwi_ctx* ctx, double* out);
+/* Copy the sources (nt, nsrc_s) or observed data (nt, nrec_s) per shot. */
+int fwi_get_sources(fwi_ctx* ctx, float* out);
+int fwi_get_observed(fwi_ctx* ctx, float* out);
+
+/* Trace conditioning with status returns (time_filter / interpolation_akima
+ * above): host arrays, column-major (nt, ntr). */
+int fwi_time_filter(float* data, int nt, float dt, int ntr, int typef, float f1, float fc);
+int fwi_resample_akima(const float* in, float dt_in, int nt_in, float* out, float dt_out, int nt_out, int ntr);
+/* Raw field data or source traces per shot, (nt_data, n_s) column-major shot
+ * after shot on their own sampling dt_data: band-passed (typef 1..3 as
+ * time_filter, 0 = no filter) and Akima-resampled onto the context's (nt, dt)
+ * on the GPU, then stored as the sources / observed data
+ * (src/calculate_misfit_gradient.f90:193-194, 230-233). */
+int fwi_set_sources_raw(fwi_ctx* ctx, const float* raw, int nt_data, float dt_data, int typef, float f1, float f2);
+int fwi_set_observed_raw(fwi_ctx* ctx, const float* raw, int nt_data, float dt_data, int typef, float f1, float f2);
 
 /* NLCG on the device-resident gradient (src/FWI.f90:146-178, 213, 315-317).
  * fwi_set_base_model: lambda0 = rho vp^2 (rho = 1000, src/auxiliar.f90:209-212)

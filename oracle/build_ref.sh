@@ -11,6 +11,7 @@
 #   _ref/libfwiref_fs1_r8.so   unmodified sources, -fdefault-real-8 (FP64 build of the same scheme)
 #   _ref/libfwiref_fs0_r4.so   free_surface=0 variant, REAL(4)
 #   _ref/libfwiref_fs0_r8.so   free_surface=0 variant, REAL(8)
+#   _ref/libfwiref_trace.so    filters_function1D + FFTPACK, INTRPL + interpolation_akima (REAL(4))
 # The free_surface=0 variant changes the one hard-coded assignment
 # `free_surface=1` (solver_acoustic_SS.f90:93) to 0 in a temporary copy made
 # outside the repository; the copy is deleted when the script exits.
@@ -60,6 +61,21 @@ build_variant() {
     echo "build_ref: built $OUT/libfwiref_$name.so"
 }
 
+# Trace conditioning (REAL(4), as the reference runs it): the FFT filter core
+# filters_function1D with the vendored FFTPACK, and the ACM 433 Akima
+# interpolation with its driver.  time_filter.f90 itself is left out: it calls
+# is_not_cero from auxiliar.f90, which needs the parfile module.
+build_trace() {
+    local dir="$TMP/trace"
+    mkdir -p "$dir"
+    for f in filters fftpack akima433 interpolation_akima; do
+        $FL -O3 -fPIC -c "$REF_SRC/$f.f90" -o "$dir/$f.o" 2>/dev/null
+    done
+    $FL -shared -fPIC -Wl,-Bsymbolic -o "$OUT/libfwiref_trace.so" "$dir"/*.o
+    echo "build_ref: built $OUT/libfwiref_trace.so"
+}
+
+build_trace
 build_variant fs1_r4 "$REF_SRC/solver_acoustic_SS.f90" ""
 build_variant fs1_r8 "$REF_SRC/solver_acoustic_SS.f90" "-fdefault-real-8"
 build_variant fs0_r4 "$TMP/solver_acoustic_SS_fs0.f90" ""

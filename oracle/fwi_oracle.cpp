@@ -746,6 +746,113 @@ void exact_adjoint(const ExactGrid<T>& g, const ExactAcq& acq, int nt, const T* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Akima interpolation, ACM Algorithm 433 as modified in akima433.f90:6-170
+// (INTRPL).  The Fortran reuses the coefficients of the previous point when
+// it falls in the same interval (label 30) and shares storage between dead
+// temporaries through EQUIVALENCE; both leave the values unchanged, so each
+// point is evaluated on its own here.  Returns the routine's ERR code.
+template <class T>
+int intrpl(int L, const T* X, const T* Y, int N, const T* U, T* V) {
+    if (L - 2 < 0) return 1;
+    if (N <= 0) return 2;
+    for (int i = 1; i < L; ++i) {
+        if (X[i - 1] - X[i] < T(0)) continue;
+        return X[i - 1] - X[i] == T(0) ? 6 : 7;
+    }
+    const int L0 = L, LM1 = L0 - 1, LM2 = LM1 - 1, LP1 = L0 + 1;
+    auto x = [&](int i) { return X[i - 1]; };   // 1-based
+    auto y = [&](int i) { return Y[i - 1]; };
+    for (int k = 0; k < N; ++k) {
+        const T UK = U[k];
+        int I;
+        if (LM2 == 0) I = 2;
+        else if (UK >= x(L0)) I = LP1;
+        else if (UK < x(1)) I = 1;
+        else {
+            int IMN = 2, IMX = L0;
+            do {
+                I = (IMN + IMX) / 2;
+                if (UK >= x(I)) IMN = I + 1;
+                else IMX = I;
+            } while (IMX > IMN);
+            I = IMX;
+        }
+        int J = I;
+        if (J == 1) J = 2;
+        if (J == LP1) J = L0;
+        T X3 = x(J - 1), Y3 = y(J - 1), X4 = x(J), Y4 = y(J);
+        T A3 = X4 - X3;
+        T M3 = (Y4 - Y3) / A3;
+        T M1 = 0, M2 = 0, M4 = 0, M5 = 0, A2 = 0, A4 = 0, X2 = 0, Y2 = 0, X5 = 0, Y5 = 0;
+        if (LM2 == 0) {
+            M2 = M3;
+            M4 = M3;
+        } else {
+            bool to41 = J == 2;
+            if (!to41) {
+                X2 = x(J - 2); Y2 = y(J - 2);
+                A2 = X3 - X2;
+                M2 = (Y3 - Y2) / A2;
+                if (J == L0) M4 = M3 + M3 - M2;
+                else to41 = true;
+            }
+            if (to41) {
+                X5 = x(J + 1); Y5 = y(J + 1);
+                A4 = X5 - X4;
+                M4 = (Y5 - Y4) / A4;
+                if (J == 2) M2 = M3 + M3 - M4;
+            }
+        }
+        if (J <= 3) M1 = M2 + M2 - M3;
+        else {
+            const T A1 = X2 - x(J - 3);
+            M1 = (Y2 - y(J - 3)) / A1;
+        }
+        if (J >= LM1) M5 = M4 + M4 - M3;
+        else {
+            const T A5 = x(J + 2) - X5;
+            M5 = (y(J + 2) - Y5) / A5;
+        }
+        T T3 = 0, T4 = 0;
+        bool lp1 = I == LP1;
+        if (!lp1) {
+            T W2 = std::fabs(M4 - M3), W3 = std::fabs(M2 - M1);
+            T SW = W2 + W3;
+            if (!(SW != T(0))) { W2 = T(0.5); W3 = T(0.5); SW = T(1); }
+            T3 = (W2 * M2 + W3 * M3) / SW;
+        }
+        if (I == 1) {
+            T4 = T3;
+            const T SA = A3 + A4;
+            T3 = T(0.5) * (M1 + M2 - A4 * (A3 - A4) * (M3 - M4) / (SA * SA));
+            X3 = X3 - A4;
+            Y3 = Y3 - M2 * A4;
+            A3 = A4;
+            M3 = M2;
+        } else {
+            T W3 = std::fabs(M5 - M4), W4 = std::fabs(M3 - M2);
+            T SW = W3 + W4;
+            if (!(SW != T(0))) { W3 = T(0.5); W4 = T(0.5); SW = T(1); }
+            T4 = (W3 * M3 + W4 * M4) / SW;
+            if (lp1) {
+                T3 = T4;
+                const T SA = A2 + A3;
+                T4 = T(0.5) * (M4 + M5 - A2 * (A2 - A3) * (M2 - M3) / (SA * SA));
+                X3 = X4;
+                Y3 = Y4;
+                A3 = A2;
+                M3 = M4;
+            }
+        }
+        const T Q2 = (T(2) * (M3 - T3) + M3 - T4) / A3;
+        const T Q3 = (-M3 - M3 + T3 + T4) / (A3 * A3);
+        const T DX = UK - X3;
+        V[k] = Y3 + DX * (T3 + DX * (Q2 + DX * Q3));
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -803,6 +910,20 @@ ORACLE_SOLVER(double, f64)
 
 ORACLE_EXACT(float, f32)
 ORACLE_EXACT(double, f64)
+
+// interpolation_akima.f90:1-31: every trace of Data_tmp(nt_Data, NumRec) on
+// the grid (i-1)*dt_Data resampled onto (i-1)*dt, nt samples.
+int oracle_akima_f32(const float* in, float dt_in, int nt_in, float* out, int ntr, float dt_out, int nt_out) {
+    std::vector<float> X(nt_in), U(nt_out);
+    for (int i = 0; i < nt_in; ++i) X[i] = float(i) * dt_in;
+    for (int i = 0; i < nt_out; ++i) U[i] = float(i) * dt_out;
+    int err = 0;
+    for (int r = 0; r < ntr; ++r) {
+        const int e = intrpl<float>(nt_in, X.data(), in + (size_t)r * nt_in, nt_out, U.data(), out + (size_t)r * nt_out);
+        if (e) err = e;
+    }
+    return err;
+}
 
 // solver_forward_simple.f90:4-542: the same scheme with dPML=50, one source
 // injected at one cell, free surface on, no snapshots.

@@ -227,3 +227,42 @@ def oracle_exact_adjoint(precision, nxsou, nysou, nxrec, nyrec, e, d, dt, vp, nt
             _ptr(srcadj), _ptr(grad) if grad is not None else None)
     assert st == 0
     return srcadj, grad
+
+
+def ref_filters_function1d(S, fe, typef, f1, fc):
+    """The reference's filters_function1D(S(ts, Num), ts, fe, Num, typef, f1, fc) (filters.f90:1), in place
+    on a copy; returns the filtered array."""
+    lib = ref("trace")
+    S = np.asfortranarray(S, dtype=np.float32).copy(order="F")
+    ts, num = S.shape
+    lib.filters_function1d_.restype = C.c_int
+    lib.filters_function1d_(_ptr(S), _iref(ts), C.byref(C.c_float(fe)), _iref(num), _iref(typef),
+                            C.byref(C.c_float(f1)), C.byref(C.c_float(fc)))
+    return S
+
+
+def ref_time_filter(data, dt, typef, f1, fc):
+    """time_filter.f90:1-38 around the reference's filters_function1D: each nonzero trace at the end of a
+    24x zero-padded record, filtered, its last nt samples kept (REAL(4) throughout)."""
+    nt, num = data.shape
+    ts = 24 * nt
+    out = np.array(data, dtype=np.float32, order="F", copy=True)
+    fe = np.float32(1.0) / np.float32(dt)
+    for i in range(num):
+        if not np.any(data[:, i] != 0):
+            continue
+        tmp = np.zeros((ts, 1), np.float32, order="F")
+        tmp[ts - nt:, 0] = data[:, i]
+        out[:, i] = ref_filters_function1d(tmp, fe, typef, f1, fc)[ts - nt:, 0]
+    return out
+
+
+def ref_akima(data, dt_in, dt_out, nt_out):
+    """The reference's interpolation_akima(Data_tmp, dt_Data, nt_Data, Data_trace, NumRec, dt, nt)."""
+    lib = ref("trace")
+    src = np.asfortranarray(data, dtype=np.float32)
+    nt_in, ntr = src.shape
+    out = np.zeros((nt_out, ntr), np.float32, order="F")
+    lib.interpolation_akima_(_ptr(src), C.byref(C.c_float(dt_in)), _iref(nt_in), _ptr(out), _iref(ntr),
+                             C.byref(C.c_float(dt_out)), _iref(nt_out))
+    return out

@@ -1,0 +1,87 @@
+"""Trace conditioning on the GPU against the oracle (and the reference build
+where present): Akima resampling bit-exact; the FFT filter within FP32 FFT
+rounding (2e-6 of the trace norm, hipFFT vs FFTPACK/numpy); the drop-in
+symbols; the context's raw-data path (filter + resample + store)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fwi_amd
+import refbind as rb
+import traces as tr
+from fwi_amd.lib import interpolation_akima, lib, time_filter
+from fwi_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(seed, nt, ntr):
+    rng = np.random.default_rng(seed)
+    d = np.cumsum(rng.standard_normal((nt, ntr)), 0).astype(np.float32)
+    d[:, 0] = synth.ricker(6.0, nt, 0.004)
+    return d
+
+
+@pytest.mark.parametrize("dt_in,dt_out,nt_in,nt_out", [(0.004, 0.0013, 300, 900), (0.002, 0.0031, 300, 250),
+                                                         (0.004, 0.001, 2, 9), (0.004, 0.002, 1250, 2500)])
+def test_akima_gpu_bit_exact(dt_in, dt_out, nt_in, nt_out):
+    d = _data(1, nt_in, 37)
+    G = interpolation_akima(d, dt_in, dt_out, nt_out)
+    O = tr.akima(rb.oracle(), d, dt_in, dt_out, nt_out)
+    assert np.array_equal(G, O)
+    if rb.ref_available("trace"):
+        assert np.array_equal(G, rb.ref_akima(d, dt_in, dt_out, nt_out))
+
+
+@pytest.mark.parametrize("typef,f1,fc", [(3, 3.0, 5.0), (3, 7.0, 10.0), (1, 0.0, 8.0), (2, 0.0, 10.0)])
+def test_filter_gpu(typef, f1, fc):
+    d = _data(2, 500, 9)
+    d[:, 3] = 0.0
+    G = time_filter(d, 0.004, typef, f1, fc)
+    O = tr.time_filter(d, 0.004, typef, f1, fc)
+    assert np.array_equal(G[:, 3], d[:, 3])
+    for j in range(d.shape[1]):
+        assert np.linalg.norm(G[:, j] - O[:, j]) <= 2e-6 * max(np.linalg.norm(d[:, j]), 1e-30), j
+    if rb.ref_available("trace"):
+        R = rb.ref_time_filter(d, 0.004, typef, f1, fc)
+        for j in range(d.shape[1]):
+            assert np.linalg.norm(G[:, j] - R[:, j]) <= 2e-6 * max(np.linalg.norm(d[:, j]), 1e-30), j
+
+
+def test_dropin_symbols():
+    d = _data(3, 300, 4)
+    L = lib()
+    r = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    f = lambda v: C.byref(C.c_float(v))  # noqa: E731
+    x = np.asfortranarray(d.copy())
+    L.time_filter_(x.ctypes.data_as(C.c_void_p), r(300), f(0.004), r(4), r(3), f(3.0), f(5.0))
+    assert np.array_equal(x, time_filter(d, 0.004, 3, 3.0, 5.0))
+    out = np.zeros((700, 4), np.float32, order="F")
+    ok = L.interpolation_akima_(np.asfortranarray(d).ctypes.data_as(C.c_void_p), f(0.004), r(300),
+                                  out.ctypes.data_as(C.c_void_p), r(4), f(0.0017), r(700))
+    assert ok == 1
+    assert np.array_equal(out, tr.akima(rb.oracle(), d, 0.004, 0.0017, 700))
+
+
+def test_context_raw_data_path():
+    """Field data and source on a 4 ms grid, band-passed 3-5 Hz and resampled onto the
+    context's 1.5 ms grid on the GPU (calculate_misfit_gradient.f90:193-194, 230-233)."""
+    ny, nx, nt, dt = 40, 80, 800, 0.0015
+    nt_d, dt_d = 320, 0.004
+    rng = np.random.default_rng(5)
+    shots = [dict(x0=0, w=60, src_x=[10], src_y=[2], rec_x=list(range(1, 61, 4)), rec_y=[2] * 15),
+             dict(x0=20, w=60, src_x=[40], src_y=[3], rec_x=list(range(2, 61, 5)), rec_y=[4] * 12)]
+    raw_obs = [rng.standard_normal((nt_d, len(s["rec_x"]))).astype(np.float32) for s in shots]
+    raw_src = [synth.ricker(4.0, nt_d, dt_d)[:, None] for _ in shots]
+    ctx = fwi_amd.Context(ny, nx, np.float32(25.0), np.float32(dt), nt, 10, dpml=20, free_surface=0)
+    ctx.set_shots(shots)
+    ctx.set_observed_raw(raw_obs, dt_d, typef=3, f1=3.0, f2=5.0)
+    ctx.set_sources_raw(raw_src, dt_d, typef=3, f1=3.0, f2=5.0)
+    for got, raw in ((ctx.observed(), raw_obs), (ctx.sources(), raw_src)):
+        for g, r in zip(got, raw):
+            filt = tr.time_filter(r, dt_d, 3, 3.0, 5.0).astype(np.float32)
+            expect = tr.akima(rb.oracle(), filt, dt_d, dt, nt)
+            scale = np.linalg.norm(tr.akima(rb.oracle(), r, dt_d, dt, nt))
+            assert np.linalg.norm(g - expect) <= 1e-5 * scale
+            assert np.abs(g).max() > 0
