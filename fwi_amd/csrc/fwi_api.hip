@@ -1449,6 +1449,53 @@ int fwi_set_observed_raw(fwi_ctx* c, const float* raw, int nt_data, float dt_dat
     return set_raw_traces(c, raw, nt_data, dt_data, typef, f1, f2, 1);
 }
 
+// source_estimation.f90:1-171 with the shot's data passed in (the reference
+// reads them with get_shot_data) and no diagnostic files: Green's functions
+// from solver_forward_simple with a 1/dt spike, windows ending half a period
+// before the water-bottom reflection, Wiener deconvolution in the band.
+int fwi_source_estimation(const float* obs, float f1, float f2, int nx, int ny, int nt, int nr, float dmodel,
+                          float dt, int nxs, int nys, const int* nxr, const int* nyr, const float* model, float* sst) {
+    if (nt < 2 || nr < 1 || nx < 1 || ny < 1) return fail("fwi_source_estimation: bad sizes");
+    if (nxs < 1 || nxs > nx) return fail("fwi_source_estimation: source column outside the model");
+    std::vector<float> s0(nt, 0.0f), Gs((size_t)nt * nr, 0.0f);
+    s0[0] = 1.0f / dt;                                            // :44
+    fwi_solver_opts o{50, 1};
+    CHK(fwi_solver_acoustic(&o, 3, 1, nr, &nxs, &nys, nxr, nyr, s0.data(), nullptr, dmodel, dt, ny, nx, model, 1,
+                            0, nt, nullptr, nullptr, Gs.data()));
+    // :79-87 water-bottom depth per column, :91-109 window ends
+    std::vector<float> zwb(nx);
+    for (int i = 0; i < nx; ++i) {
+        zwb[i] = (float)(ny - 1) * dmodel;
+        for (int j = 0; j < ny; ++j)
+            if (model[(size_t)i * ny + j] > 1500.0f) { zwb[i] = (float)j * dmodel; break; }
+    }
+    const float fc = (f1 + f2) / 2.0f;
+    std::vector<int> it2(nr);
+    for (int r = 0; r < nr; ++r) {
+        if (nxr[r] < 1 || nxr[r] > nx) return fail("fwi_source_estimation: receiver column outside the model");
+        const float xmid = 0.5f * (float)(nxr[r] + nxs) * dmodel;
+        const int ixm = (int)(xmid / dmodel) + 1;
+        const float z = zwb[std::min(std::max(ixm, 1), nx) - 1];
+        const float h = (float)std::abs(nxr[r] - nxs) * dmodel * 0.5f;
+        const float t_wb = 2.0f * std::sqrt(h * h + z * z) / 1500.0f;
+        const float t2 = t_wb - 0.5f / fc;
+        it2[r] = std::max(0, std::min(nt, 1 + (int)(t2 / dt)));
+    }
+    float *dGs = nullptr, *dGo = nullptr;
+    const size_t bytes = (size_t)nt * nr * sizeof(float);
+    HIPCHK(hipMalloc(&dGs, bytes));
+    int rc = 0;
+    std::string err;
+    if (hipMalloc(&dGo, bytes) != hipSuccess) rc = fail("fwi_source_estimation: allocation failed");
+    if (!rc && (hipMemcpy(dGs, Gs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dGo, obs, bytes, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail("fwi_source_estimation: upload failed");
+    if (!rc && fwi::wiener_source(dGs, dGo, nt, nr, dt, f1, f2, it2.data(), sst, nullptr, &err)) rc = fail(err);
+    (void)hipFree(dGs);
+    if (dGo) (void)hipFree(dGo);
+    return rc;
+}
+
 int fwi_set_schedule(fwi_ctx* c, int segment_slots) {
     if (!c) return fail("null context");
     if (segment_slots < 0) return fail("segment_slots must be >= 0");

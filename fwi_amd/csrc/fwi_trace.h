@@ -17,6 +17,11 @@ int trace_filter(float* traces, int nt, int ntr, float dt, int typef, float f1, 
 // Akima resampling of device traces (nt_in, ntr) on (i-1) dt_in onto (i-1) dt_out, (nt_out, ntr).
 hipError_t akima_resample(const float* in, float dt_in, int nt_in, float* out, float dt_out, int nt_out, int ntr,
                           hipStream_t st);
+// Wiener estimate of the source (source_estimation.f90:79-167): Gs, Go device
+// (nt, nr) column-major, windowed in place after it2_host[r] samples and
+// normalised; sst (host, nt) receives the wavelet.  Returns 0 or -1.
+int wiener_source(float* Gs, float* Go, int nt, int nr, float dt, float f1, float f2, const int* it2_host,
+                  float* sst, hipStream_t st, std::string* err);
 // (nt, ntr) column-major -> time-major [t][ntr]
 hipError_t traces_to_time_major(const float* in, int nt, int ntr, float* out, hipStream_t st);
 

@@ -191,9 +191,144 @@ __global__ void time_major_kernel(const float* __restrict__ in, int nt, int ntr,
     }
 }
 
+// ---- Wiener source estimation (source_estimation.f90) ----
+__global__ void window_maxabs_kernel(float* __restrict__ G, int nt, int nr, const int* __restrict__ it2,
+                                     unsigned int* __restrict__ maxbits) {
+    const size_t total = (size_t)nr * nt;
+    float m = 0.0f;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / nt), i = (int)(e % nt);
+        float v = G[e];
+        if (i >= it2[r]) { v = 0.0f; G[e] = 0.0f; }
+        m = fmaxf(m, fabsf(v));
+    }
+    atomicMax(maxbits, __float_as_uint(m));
+}
+
+// complex records [2 nr][npad]: Gs traces, then Go traces, each divided by its max |value|, zero padded
+__global__ void pack_kernel(const float* __restrict__ Gs, const float* __restrict__ Go, int nt, int nr, int npad,
+                            const unsigned int* __restrict__ maxbits, float2* __restrict__ rec) {
+    const size_t total = (size_t)2 * nr * npad;
+    const float ms = __uint_as_float(maxbits[0]), mo = __uint_as_float(maxbits[1]);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / npad), i = (int)(e % npad);
+        float v = 0.0f;
+        if (i < nt) v = t < nr ? Gs[(size_t)t * nt + i] / ms : Go[(size_t)(t - nr) * nt + i] / mo;
+        rec[e] = make_float2(v, 0.0f);
+    }
+}
+
+// kk = sum_r |Ws|^2, bb = sum_r conj(Ws) Wo over bins [b0, b1), spectra scaled by 1/npad, FP64
+__global__ void band_sums_kernel(const float2* __restrict__ W, int nr, int npad, int b0, int b1,
+                                 double* __restrict__ kk, double2* __restrict__ bb) {
+    const int b = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= b1) return;
+    const double inv = 1.0 / (double)npad;
+    double k = 0.0, br = 0.0, bi = 0.0;
+    for (int r = 0; r < nr; ++r) {
+        const float2 s = W[(size_t)r * npad + b], o = W[(size_t)(nr + r) * npad + b];
+        const double sr = s.x * inv, si = s.y * inv, orr = o.x * inv, oi = o.y * inv;
+        k += sr * sr + si * si;
+        br += sr * orr + si * oi;
+        bi += sr * oi - si * orr;
+    }
+    kk[b - b0] = k;
+    bb[b - b0] = make_double2(br, bi);
+}
+
 unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>(16384, std::max<size_t>(1, (n + 255) / 256)); }
 
 }  // namespace
+
+int wiener_source(float* Gs, float* Go, int nt, int nr, float dt, float f1, float f2, const int* it2_host,
+                  float* sst, hipStream_t st, std::string* err) {
+    const int npad = 2 * nt;
+    const float df = 1.0f / ((float)nt * dt);
+    const int if1 = 1 + (int)std::floor(f1 / df), if2 = 1 + (int)std::floor(f2 / df);
+    const int b0 = std::max(if1 - 1, 0), b1 = std::min(if2, npad);
+    int status = 0;
+    auto fail = [&](const std::string& m) { if (err) *err = m; status = -1; };
+    int* it2 = nullptr;
+    unsigned int* maxbits = nullptr;
+    float2* rec = nullptr;
+    double* kk = nullptr;
+    double2* bb = nullptr;
+    hipfftHandle plan = 0, plan1 = 0;
+    bool have = false, have1 = false;
+    do {
+        if (b1 <= b0) { fail("source estimation: empty frequency band"); break; }
+        if (hipMalloc(&it2, nr * sizeof(int)) != hipSuccess || hipMalloc(&maxbits, 2 * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&rec, (size_t)2 * nr * npad * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&kk, (size_t)(b1 - b0) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&bb, (size_t)(b1 - b0) * sizeof(double2)) != hipSuccess) {
+            fail("source estimation: device allocation failed");
+            break;
+        }
+        (void)hipMemcpyAsync(it2, it2_host, nr * sizeof(int), hipMemcpyHostToDevice, st);
+        (void)hipMemsetAsync(maxbits, 0, 2 * sizeof(unsigned), st);
+        hipLaunchKernelGGL(window_maxabs_kernel, dim3(blocks_for((size_t)nr * nt)), dim3(256), 0, st, Gs, nt, nr, it2,
+                           maxbits);
+        hipLaunchKernelGGL(window_maxabs_kernel, dim3(blocks_for((size_t)nr * nt)), dim3(256), 0, st, Go, nt, nr, it2,
+                           maxbits + 1);
+        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for((size_t)2 * nr * npad)), dim3(256), 0, st, Gs, Go, nt, nr,
+                           npad, maxbits, rec);
+        int n = npad;
+        if (hipfftPlanMany(&plan, 1, &n, nullptr, 1, npad, nullptr, 1, npad, HIPFFT_C2C, 2 * nr) != HIPFFT_SUCCESS) {
+            fail("source estimation: hipfftPlanMany failed");
+            break;
+        }
+        have = true;
+        hipfftSetStream(plan, st);
+        if (hipfftExecC2C(plan, reinterpret_cast<hipfftComplex*>(rec), reinterpret_cast<hipfftComplex*>(rec),
+                          HIPFFT_FORWARD) != HIPFFT_SUCCESS) {
+            fail("source estimation: forward FFT failed");
+            break;
+        }
+        hipLaunchKernelGGL(band_sums_kernel, dim3((b1 - b0 + 255) / 256), dim3(256), 0, st, rec, nr, npad, b0, b1,
+                           kk, bb);
+        std::vector<double> hk(b1 - b0);
+        std::vector<double2> hb(b1 - b0);
+        if (hipMemcpyAsync(hk.data(), kk, hk.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hb.data(), bb, hb.size() * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            fail("source estimation: download failed");
+            break;
+        }
+        double mx = hk[0];
+        for (double v : hk) mx = std::max(mx, v);
+        const double eps = 0.001 * mx;
+        std::vector<float2> ss(npad, float2{0.0f, 0.0f});
+        for (int b = b0; b < b1; ++b) {
+            const std::complex<double> q =
+                std::complex<double>(hb[b - b0].x, hb[b - b0].y) / std::complex<double>(hk[b - b0] + eps, 0.0);
+            ss[b] = float2{(float)q.real(), (float)q.imag()};
+        }
+        if (hipfftPlan1d(&plan1, npad, HIPFFT_C2C, 1) != HIPFFT_SUCCESS) {
+            fail("source estimation: hipfftPlan1d failed");
+            break;
+        }
+        have1 = true;
+        hipfftSetStream(plan1, st);
+        if (hipMemcpyAsync(rec, ss.data(), npad * sizeof(float2), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipfftExecC2C(plan1, reinterpret_cast<hipfftComplex*>(rec), reinterpret_cast<hipfftComplex*>(rec),
+                          HIPFFT_BACKWARD) != HIPFFT_SUCCESS) {
+            fail("source estimation: inverse FFT failed");
+            break;
+        }
+        std::vector<float2> out(nt);
+        if (hipMemcpyAsync(out.data(), rec, nt * sizeof(float2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            fail("source estimation: download failed");
+            break;
+        }
+        for (int i = 0; i < nt; ++i) sst[i] = out[i].x / (float)npad;
+    } while (false);
+    if (have) hipfftDestroy(plan);
+    if (have1) hipfftDestroy(plan1);
+    for (void* p : {(void*)it2, (void*)maxbits, (void*)rec, (void*)kk, (void*)bb})
+        if (p) (void)hipFree(p);
+    return status;
+}
 
 hipError_t traces_to_time_major(const float* in, int nt, int ntr, float* out, hipStream_t st) {
     if (ntr <= 0 || nt <= 0) return hipSuccess;

@@ -59,7 +59,8 @@ def lib():
                      "fwi_set_base_model", "fwi_nlcg_direction", "fwi_search_scale", "fwi_apply_step",
                      "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient",
                      "fwi_autotune", "fwi_get_rows_per_task", "fwi_get_sources", "fwi_get_observed",
-                     "fwi_time_filter", "fwi_resample_akima", "fwi_set_sources_raw", "fwi_set_observed_raw"):
+                     "fwi_time_filter", "fwi_resample_akima", "fwi_set_sources_raw", "fwi_set_observed_raw",
+                     "fwi_source_estimation"):
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
@@ -177,6 +178,23 @@ def interpolation_akima(data, dt_data, dt, nt):
     out = np.zeros((nt, ntr), np.float32, order="F")
     _check(lib().fwi_resample_akima(_ptr(d), float(dt_data), nt_in, _ptr(out), float(dt), int(nt), ntr),
            "fwi_resample_akima")
+    return out
+
+
+def source_estimation(obs, f1, f2, dmodel, dt, nxs, nys, nxRec, nyRec, model):
+    """Mirror of the reference's source_estimation (source_estimation.f90:1) with the shot's observed
+    data obs (nt, nr) passed in; returns the estimated wavelet sst (nt,)."""
+    obs = _f32f(obs)
+    model = _f32f(model)
+    nt, nr = obs.shape
+    ny, nx = model.shape
+    nxr, nyr = _i32(nxRec), _i32(nyRec)
+    out = np.zeros(nt, np.float32)
+    lib().fwi_source_estimation.argtypes = [_P, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                            C.c_float, C.c_int, C.c_int, _P, _P, _P, _P]
+    _check(lib().fwi_source_estimation(_ptr(obs), float(f1), float(f2), nx, ny, nt, nr, float(dmodel), float(dt),
+                                       int(nxs), int(nys), _ptr(nxr), _ptr(nyr), _ptr(model), _ptr(out)),
+           "fwi_source_estimation")
     return out
 
 

@@ -180,6 +180,17 @@ int fwi_get_observed(fwi_ctx* ctx, float* out);
  * above): host arrays, column-major (nt, ntr). */
 int fwi_time_filter(float* data, int nt, float dt, int ntr, int typef, float f1, float fc);
 int fwi_resample_akima(const float* in, float dt_in, int nt_in, float* out, float dt_out, int nt_out, int ntr);
+/* Source estimation (src/source_estimation.f90:1, called at src/FWI.f90:101
+ * when inv_source = 1) with the shot's observed data obs(nt, nr) passed in
+ * (the reference reads them through get_shot_data) and without its
+ * diagnostic files: Green's functions of a 1/dt spike from the dPML = 50
+ * forward solver on the GPU, windows ending half a period before the
+ * water-bottom reflection, Wiener deconvolution sum_r conj(Gs) Go /
+ * (sum_r |Gs|^2 + 1e-3 max) on the band's bins with batched FFTs on the GPU
+ * (FP64 band sums).  sst(nt) receives the wavelet. */
+int fwi_source_estimation(const float* obs, float f1, float f2, int nx, int ny, int nt, int nr, float dmodel,
+                          float dt, int nxs, int nys, const int* nxr, const int* nyr, const float* model, float* sst);
+
 /* Raw field data or source traces per shot, (nt_data, n_s) column-major shot
  * after shot on their own sampling dt_data: band-passed (typef 1..3 as
  * time_filter, 0 = no filter) and Akima-resampled onto the context's (nt, dt)

@@ -82,3 +82,65 @@ def akima(lib, data: np.ndarray, dt_in: float, dt_out: float, nt_out: int) -> np
              C.c_int(ntr), C.c_float(dt_out), C.c_int(nt_out))
     assert err == 0, err
     return out
+
+
+def _estimation_windows(nx, ny, nt, dmodel, dt, nxs, nxr, model, f1, f2):
+    """Per-receiver last kept sample it2 (source_estimation.f90:79-109) in REAL(4): the water bottom is the
+    first row with vp > 1500 under the source-receiver midpoint, the window ends half a period before the
+    water-bottom reflection."""
+    f32 = np.float32
+    d = f32(dmodel)
+    zwb = np.full(nx, f32(ny - 1) * d, np.float32)
+    for i in range(nx):
+        rows = np.nonzero(model[:, i] > f32(1500.0))[0]
+        if len(rows):
+            zwb[i] = f32(rows[0]) * d
+    fc = (f32(f1) + f32(f2)) / f32(2.0)
+    it2 = np.zeros(len(nxr), np.int64)
+    for r, x in enumerate(nxr):
+        xmid = f32(0.5) * f32(int(x) + int(nxs)) * d
+        ixm = int(xmid / d) + 1
+        z = zwb[ixm - 1]
+        h = f32(abs(int(x) - int(nxs))) * d * f32(0.5)
+        t_wb = f32(2.0) * np.sqrt(h * h + z * z) / f32(1500.0)
+        t2 = t_wb - f32(0.5) / fc
+        it2[r] = max(0, min(nt, 1 + int(t2 / f32(dt))))
+    return it2
+
+
+def wiener_source(Gs, Go, nt, dt, f1, f2):
+    """Frequency-domain core of source_estimation.f90:120-167 in FP64: 2x padded FFTs (FFTPACK's forward
+    1/N), kk = sum |Gs|^2 and bb = sum conj(Gs) Go over receivers on bins if1..if2 (bin spacing taken as
+    1/(nt dt) as the reference does on its 2nt-point transform), s = bb / (kk + 1e-3 max kk), inverse
+    transform, real part / (2 nt)."""
+    f32 = np.float32
+    npad = 2 * nt
+    df = f32(1.0) / (f32(nt) * f32(dt))
+    if1, if2 = 1 + int(np.floor(f32(f1) / df)), 1 + int(np.floor(f32(f2) / df))
+    Ws = np.fft.fft(Gs.astype(np.float64), n=npad, axis=0) / npad
+    Wo = np.fft.fft(Go.astype(np.float64), n=npad, axis=0) / npad
+    band = slice(if1 - 1, if2)
+    kk = (np.conj(Ws[band]) * Ws[band]).sum(1)
+    bb = (np.conj(Ws[band]) * Wo[band]).sum(1)
+    eps = 0.001 * kk.real.max()
+    ss = np.zeros(npad, np.complex128)
+    ss[band] = bb / (kk + eps)
+    return (np.fft.ifft(ss) * npad).real[:nt] / npad
+
+
+def source_estimation(obs, f1, f2, nx, ny, nt, nr, dmodel, dt, nxs, nys, nxr, nyr, model, forward_simple):
+    """source_estimation.f90:1-171 with the observed data passed in (the reference reads them with
+    get_shot_data) and without its diagnostic file output.  forward_simple(nxs, nys, nxr, nyr, S, d, dt,
+    model, nt) -> (nt, nr) is the modelling routine (solver_forward_simple)."""
+    f32 = np.float32
+    s0 = np.zeros(nt, np.float32)
+    s0[0] = f32(1.0) / f32(dt)
+    Gs = np.array(forward_simple(nxs, nys, nxr, nyr, s0, f32(dmodel), f32(dt), model, nt), dtype=np.float32)
+    Go = np.array(obs, dtype=np.float32, copy=True)
+    it2 = _estimation_windows(nx, ny, nt, dmodel, dt, nxs, nxr, model, f1, f2)
+    for r in range(nr):
+        Gs[it2[r]:, r] = 0.0
+        Go[it2[r]:, r] = 0.0
+    Gs = Gs / np.abs(Gs).max()
+    Go = Go / np.abs(Go).max()
+    return wiener_source(Gs, Go, nt, dt, f1, f2)

@@ -87,3 +87,37 @@ def test_filter_linearity():
     a = tr.time_filter(d, 0.004, 3, 3.0, 5.0)
     b = tr.time_filter(np.float32(2.0) * d, 0.004, 3, 3.0, 5.0)
     assert np.allclose(b, 2.0 * a, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
+def test_wiener_core_identities():
+    """Identical data give s(w) = kk/(kk+eps) on the band: a zero-phase pulse at t = 0 of height
+    sum_band kk/(kk+eps) / npad; data delayed by m samples move the pulse to t = m."""
+    rng = np.random.default_rng(7)
+    nt, nr, dt, f1, f2 = 400, 6, 0.004, 3.0, 9.0
+    Gs = np.zeros((nt, nr), np.float32)
+    Gs[20:120] = rng.standard_normal((100, nr))
+    s = tr.wiener_source(Gs, Gs, nt, dt, f1, f2)
+    npad = 2 * nt
+    df = np.float32(1.0) / (np.float32(nt) * np.float32(dt))
+    if1, if2 = 1 + int(np.floor(np.float32(f1) / df)), 1 + int(np.floor(np.float32(f2) / df))
+    W = np.fft.fft(Gs.astype(np.float64), n=npad, axis=0)[if1 - 1:if2] / npad
+    kk = (np.abs(W) ** 2).sum(1)
+    expect0 = (kk / (kk + 0.001 * kk.max())).sum() / npad
+    assert s[0] == pytest.approx(expect0, rel=1e-12)
+    assert np.argmax(s) == 0
+    m = 37
+    Go = np.zeros_like(Gs)
+    Go[m:] = Gs[:-m]
+    assert np.argmax(tr.wiener_source(Gs, Go, nt, dt, f1, f2)) == m
+
+
+def test_estimation_windows():
+    """Water 1500 m/s down to row 9 (z = 200 m at d = 25 m): the window of a receiver at offset h
+    ends half a period before the water-bottom reflection 2 sqrt((h/2)^2 + 200^2)/1500."""
+    ny, nx, nt, d, dt = 30, 60, 500, 25.0, 0.002
+    model = np.full((ny, nx), 1500.0, np.float32)
+    model[8:, :] = 2000.0
+    it2 = tr._estimation_windows(nx, ny, nt, d, dt, 20, np.array([20, 40, 60]), model, 3.0, 5.0)
+    for r, x in enumerate((20, 40, 60)):
+        t2 = 2 * np.sqrt((abs(x - 20) * d / 2) ** 2 + 200.0 ** 2) / 1500.0 - 0.5 / 4.0
+        assert abs(it2[r] - (1 + int(t2 / dt))) <= 1
