@@ -64,6 +64,10 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
+        L.fwi_su_last_error.restype = C.c_char_p
+        for name in ("fwi_su_info", "fwi_su_read", "fwi_su_header_word", "fwi_su_write", "fwi_akima"):
+            getattr(L, name).restype = C.c_int
+        L.fwi_su_write.argtypes = [C.c_char_p, C.c_int, _P, C.c_int, C.c_int, C.c_float]
         L.time_filter_.restype = None
         L.interpolation_akima_.restype = C.c_int
         L.fwi_time_filter.argtypes = [_P, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_float]
@@ -196,6 +200,55 @@ def source_estimation(obs, f1, f2, dmodel, dt, nxs, nys, nxRec, nyRec, model):
                                        int(nxs), int(nys), _ptr(nxr), _ptr(nyr), _ptr(model), _ptr(out)),
            "fwi_source_estimation")
     return out
+
+
+def _su_check(rc, what):
+    if rc != 0:
+        raise FwiError(f"{what}: {lib().fwi_su_last_error().decode()}")
+
+
+def su_info(path, big_endian=True):
+    """(ns, ntr, dt in seconds) of an SU file."""
+    ns, ntr, dt = C.c_int(0), C.c_int(0), C.c_int(0)
+    _su_check(lib().fwi_su_info(os.fsencode(path), int(big_endian), C.byref(ns), C.byref(ntr), C.byref(dt)),
+              "fwi_su_info")
+    return ns.value, ntr.value, dt.value * 1e-6
+
+
+def su_read(path, big_endian=True, first=0, ntr=None, ns=None):
+    """Traces of an SU file as a (ns, ntr) float32 array (get_shot_data / get_su_data)."""
+    n, total, _ = su_info(path, big_endian)
+    ns = ns or n
+    ntr = total - first if ntr is None else ntr
+    out = np.zeros((ns, ntr), np.float32, order="F")
+    _su_check(lib().fwi_su_read(os.fsencode(path), int(big_endian), ns, int(first), ntr, _ptr(out)), "fwi_su_read")
+    return out
+
+
+def su_header_word(path, trace, byte_pos, nbytes=4, big_endian=True, ns=None):
+    """Signed header value at 1-based byte byte_pos of a trace (0-based)."""
+    ns = ns or su_info(path, big_endian)[0]
+    v = C.c_longlong(0)
+    _su_check(lib().fwi_su_header_word(os.fsencode(path), int(big_endian), ns, int(trace), int(byte_pos), int(nbytes),
+                                       C.byref(v)), "fwi_su_header_word")
+    return v.value
+
+
+def su_write(path, data, dt, big_endian=True):
+    """Write (ns, ntr) traces with tracl, ns and dt headers (write_new_data's output format)."""
+    d = _f32f(data)
+    ns, ntr = d.shape
+    _su_check(lib().fwi_su_write(os.fsencode(path), int(big_endian), _ptr(d), ns, ntr, float(dt)), "fwi_su_write")
+
+
+def akima(X, Y, U):
+    """INTRPL (akima433.f90) on host: values at U of the Akima interpolant of (X, Y)."""
+    X, Y, U = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, U))
+    V = np.zeros(len(U), np.float32)
+    err = lib().fwi_akima(len(X), _ptr(X), _ptr(Y), len(U), _ptr(U), _ptr(V))
+    if err:
+        raise FwiError(f"fwi_akima: INTRPL error {err}")
+    return V
 
 
 def beta_calculous(Grad, OldGrad):

@@ -265,6 +265,27 @@ int fwi_set_graphs(fwi_ctx* ctx, int enable);
 int fwi_set_timing(fwi_ctx* ctx, int enable);
 int fwi_get_timing(fwi_ctx* ctx, double* step_kernel_ms, long long* step_launches, int reset);
 
+/* SU files (src/auxiliar.f90:9-50, src/get_mcs_data.f90:207-240, 291-326):
+ * traces of a 240-byte header (ns at bytes 115-116, dt in microseconds at
+ * 117-118) and ns float32 samples, little- (big_endian = 0) or big-endian.
+ * fwi_su_info reads ns, dt and the trace count; fwi_su_read reads ntr traces
+ * from trace `first` (0-based) into data(ns, ntr); fwi_su_header_word reads
+ * a 2- or 4-byte signed header value at 1-based byte byte_pos of a trace (the
+ * shot number / offset reads of get_pos_channels); fwi_su_write writes
+ * data(ns, ntr) with tracl = 1..ntr, ns and dt = int(dt * 1e6) in the
+ * headers (the reference's suaddhead | sushw output).  Host-side; status 0
+ * or -1 with fwi_su_last_error(). */
+const char* fwi_su_last_error(void);
+int fwi_su_info(const char* path, int big_endian, int* ns, int* ntr, int* dt_us);
+int fwi_su_read(const char* path, int big_endian, int ns, int first, int ntr, float* data);
+int fwi_su_header_word(const char* path, int big_endian, int ns, int trace, int byte_pos, int nbytes,
+                       long long* value);
+int fwi_su_write(const char* path, int big_endian, const float* data, int ns, int ntr, float dt);
+/* ACM 433 Akima interpolation (src/akima433.f90 INTRPL) of L points with
+ * increasing abscissae X onto N abscissae U (host; used for the bathymetry
+ * model of src/get_mcs_data.f90:376-426).  Returns INTRPL's ERR code. */
+int fwi_akima(int L, const float* X, const float* Y, int N, const float* U, float* V);
+
 /* Synchronise the context's stream. */
 int fwi_sync(fwi_ctx* ctx);
 /* Device bytes held by the context. */

@@ -911,6 +911,11 @@ ORACLE_SOLVER(double, f64)
 ORACLE_EXACT(float, f32)
 ORACLE_EXACT(double, f64)
 
+// akima433.f90 INTRPL on arbitrary increasing abscissae.
+int oracle_intrpl_f32(int L, const float* X, const float* Y, int N, const float* U, float* V) {
+    return intrpl<float>(L, X, Y, N, U, V);
+}
+
 // interpolation_akima.f90:1-31: every trace of Data_tmp(nt_Data, NumRec) on
 // the grid (i-1)*dt_Data resampled onto (i-1)*dt, nt samples.
 int oracle_akima_f32(const float* in, float dt_in, int nt_in, float* out, int ntr, float dt_out, int nt_out) {

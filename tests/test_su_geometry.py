@@ -1,0 +1,143 @@
+"""SU I/O, host Akima interpolation and the MCS geometry set-up (host code of
+libfwi_hip.so and fwi_amd/geometry.py; no GPU needed).
+
+* SU files written by the library parse independently (struct) as 240-byte
+  headers + float32 samples with tracl, ns and dt set, in either byte order,
+  and read back unchanged; hand-built files read correctly, header words
+  included; a truncated file is rejected.
+* fwi_akima (general abscissae) is bit-identical to the reference's INTRPL
+  (oracle/_ref) and to the oracle's restatement.
+* Geometry: shot and receiver grid indices follow 1 + ceiling(x/d) from the
+  navigation file and the SU offsets, the bathymetry model equals the
+  reference's INTRPL on the same points."""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import refbind as rb
+from fwi_amd import FwiError
+from fwi_amd.geometry import MCSGeometry, grid
+from fwi_amd.lib import akima, su_header_word, su_info, su_read, su_write
+
+
+def _parse(path, big):
+    e = ">" if big else "<"
+    raw = open(path, "rb").read()
+    ns = struct.unpack(e + "h", raw[114:116])[0]
+    dt = struct.unpack(e + "h", raw[116:118])[0]
+    tb = 240 + 4 * ns
+    ntr = len(raw) // tb
+    data = np.zeros((ns, ntr), np.float32)
+    tracl = []
+    for j in range(ntr):
+        tracl.append(struct.unpack(e + "i", raw[j * tb:j * tb + 4])[0])
+        data[:, j] = np.frombuffer(raw[j * tb + 240:(j + 1) * tb], dtype=np.dtype(np.float32).newbyteorder(e))
+    return ns, dt, tracl, data
+
+
+@pytest.mark.parametrize("big", [True, False])
+def test_su_write_read_roundtrip(tmp_path, big):
+    d = np.random.default_rng(0).standard_normal((333, 7)).astype(np.float32)
+    p = str(tmp_path / "t.su")
+    su_write(p, d, 0.004, big_endian=big)
+    ns, dt, tracl, data = _parse(p, big)
+    assert ns == 333 and dt == 4000 and tracl == list(range(1, 8))
+    assert np.array_equal(data, d)
+    assert su_info(p, big) == (333, 7, pytest.approx(0.004))
+    assert np.array_equal(su_read(p, big), d)
+    assert np.array_equal(su_read(p, big, first=2, ntr=3), d[:, 2:5])
+
+
+def test_su_read_hand_built(tmp_path):
+    ns, ntr = 50, 4
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal((ns, ntr)).astype(np.float32)
+    p = str(tmp_path / "h.su")
+    with open(p, "wb") as fh:
+        for j in range(ntr):
+            h = bytearray(240)
+            struct.pack_into(">i", h, 8, 1234)                # fldr (bytes 9-12): shot number
+            struct.pack_into(">i", h, 36, -(100 + 25 * j))    # offset (bytes 37-40)
+            struct.pack_into(">h", h, 114, ns)
+            struct.pack_into(">h", h, 116, 2000)
+            fh.write(bytes(h) + d[:, j].astype(">f4").tobytes())
+    assert np.array_equal(su_read(p, True), d)
+    assert su_header_word(p, 0, 9) == 1234
+    assert [su_header_word(p, j, 37) for j in range(ntr)] == [-(100 + 25 * j) for j in range(ntr)]
+    with open(p, "ab") as fh:
+        fh.write(b"\0" * 10)
+    with pytest.raises(FwiError):
+        su_info(p, True)
+
+
+def _ref_intrpl(X, Y, U):
+    lib = rb.ref("trace")
+    X, Y, U = (np.ascontiguousarray(a, np.float32) for a in (X, Y, U))
+    V = np.zeros(len(U), np.float32)
+    err = C.c_int(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    lib.intrpl_(C.byref(C.c_int(len(X))), p(X), p(Y), C.byref(C.c_int(len(U))), p(U), p(V), C.byref(err))
+    assert err.value == 0
+    return V
+
+
+def _oracle_intrpl(X, Y, U):
+    X, Y, U = (np.ascontiguousarray(a, np.float32) for a in (X, Y, U))
+    V = np.zeros(len(U), np.float32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    assert rb.oracle().oracle_intrpl_f32(len(X), p(X), p(Y), len(U), p(U), p(V)) == 0
+    return V
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_host_akima_nonuniform(seed):
+    rng = np.random.default_rng(seed)
+    X = np.cumsum(rng.uniform(10, 80, 25)).astype(np.float32)
+    Y = rng.uniform(1500, 3000, 25).astype(np.float32)
+    U = np.linspace(X[0] - 50, X[-1] + 50, 700).astype(np.float32)
+    V = akima(X, Y, U)
+    assert np.array_equal(V, _oracle_intrpl(X, Y, U))
+    if rb.ref_available("trace"):
+        assert np.array_equal(V, _ref_intrpl(X, Y, U))
+
+
+def test_geometry_from_navigation_and_su(tmp_path):
+    nav = tmp_path / "nav.txt"
+    xs = [0.0, 37.5, 75.0, 112.5, 150.0]
+    bats = [-1500.0, -1600.0, -1750.0, -1800.0, -1790.0]
+    nav.write_text("".join(f"{100 + i} {x} {b}\n" for i, (x, b) in enumerate(zip(xs, bats))))
+    d, add = 25.0, 1000.0
+    g = MCSGeometry.from_navigation(str(nav), 5, d, add)
+    assert list(g.shot_id) == [100, 101, 102, 103, 104]
+    assert list(g.pos_shot_grid) == [1 + int(np.ceil(np.float32(add + x) / np.float32(d))) for x in xs]
+    assert list(g.pos_bat_grid) == [1 + int(np.ceil(np.float32(-b) / np.float32(d))) for b in bats]
+    assert g.nxmodel == 1 + int(np.ceil(np.float32(150.0 + 2 * add) / np.float32(d)))
+    g.receivers_nominal(6, 150.0, 12.5)
+    assert g.pos_rec_grid.shape == (6, 5)
+    assert g.pos_rec_grid[3, 2] == 1 + int(np.ceil(np.float32(add + 75.0 - 150.0 - 3 * 12.5) / np.float32(d)))
+    # receivers from the SU offsets (4-byte offset at byte 37, in decimetres: unit -10)
+    files = []
+    for s in range(5):
+        p = str(tmp_path / f"s{s}.su")
+        with open(p, "wb") as fh:
+            for r in range(6):
+                h = bytearray(240)
+                struct.pack_into(">i", h, 36, 1500 + 125 * r)
+                struct.pack_into(">h", h, 114, 8)
+                fh.write(bytes(h) + np.zeros(8, ">f4").tobytes())
+        files.append(p)
+    g2 = MCSGeometry.from_navigation(str(nav), 5, d, add).receivers_from_su(files, 6, 37, -10)
+    assert np.array_equal(g2.pos_rec, g.pos_rec)
+    nxs, nys, nxr, nyr = g.source_receiver_indices(10.0, 12.0)
+    assert nys == 2 and nyr == 2 and np.array_equal(nxr, g.pos_rec_grid)
+    bat, bgrid = g.bathymetry(g.nxmodel, g.length_model)
+    X = np.concatenate([[0.0], g.pos_shot, [g.length_model]]).astype(np.float32)
+    Y = np.concatenate([[g.pos_bat[0]], g.pos_bat, [g.pos_bat[-1]]]).astype(np.float32)
+    xm = np.arange(g.nxmodel, dtype=np.float32) * np.float32(d)
+    assert np.array_equal(bat, _oracle_intrpl(X, Y, xm))
+    if rb.ref_available("trace"):
+        assert np.array_equal(bat, _ref_intrpl(X, Y, xm))
+    assert np.array_equal(bgrid, grid(bat / np.float32(d)))
