@@ -8,6 +8,7 @@
 // translate between the reference's host array layouts and the device
 // layouts.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -123,6 +124,21 @@ struct fwi_ctx {
     fwi::ShotDev* d_desc = nullptr;
     fwi::Task* d_tasks = nullptr;         // frame and interior tasks, grouped per XCD
     int ntasks = 0;
+    // Shot groups: the sweeps advance one group of contiguous shots through
+    // all its steps before the next, so a group's propagation state stays in
+    // the GPU's last-level cache between steps.  want_groups = 0 sizes the
+    // groups to kGroupStateBytes of state each.
+    struct ShotGroup {
+        int s0 = 0, s1 = 0;               // shots [s0, s1)
+        size_t st0 = 0, st1 = 0;          // their bytes [st0, st1) of the state arena
+        fwi::Task* d_tasks = nullptr;
+        int ntasks = 0;
+    };
+    std::vector<ShotGroup> sgroups;
+    int want_groups = 0;
+    Arena gtasks;
+    std::vector<std::array<std::vector<fwi::Task>, 3>> shot_kinds;   // per shot: frame, y-band, interior tasks
+    std::vector<long long> shot_work;
 
     int* d_rows = nullptr;
     int2* d_ents = nullptr;
@@ -193,6 +209,8 @@ struct fwi_ctx {
     void release() {
         release_storage();
         state.release(); stat.release(); traces.release(); grads.release(); misc.release(); exact.release();
+        gtasks.release();
+        sgroups.clear();
         exact_hist = false;
         have_shots = have_model = have_synth = have_snaps = false;
     }
@@ -274,6 +292,55 @@ void attach_cells(fwi::Task& t, int NX, const std::vector<Cell>& cells, std::vec
     }
     rows.push_back((int)ents.size());
 }
+
+// Task list of a set of shots for one launch.  XCD groups: block b runs on
+// XCD b % 8 and takes task (b % 8) * per + b / 8, so XCD group g is a
+// contiguous range of `per` tasks.  Shots go to XCD groups largest first
+// onto the least-loaded one; within an XCD group the task kinds are
+// interleaved in proportion (a latency-bound frame task next to
+// bandwidth-bound interior tasks keeps every CU busy on both).  Padded with
+// TASK_IDLE to a multiple of 8.
+std::vector<fwi::Task> xcd_task_list(const std::vector<std::array<std::vector<fwi::Task>, 3>>& kinds_of,
+                                     const std::vector<long long>& work, const std::vector<int>& shots) {
+    std::vector<int> order = shots;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<std::vector<int>> xg(8);
+    std::vector<long long> load(8, 0);
+    for (int i : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        xg[g].push_back(i);
+        load[g] += work[i];
+    }
+    std::vector<std::vector<fwi::Task>> gt(8);
+    size_t per = 0;
+    for (int g = 0; g < 8; ++g) {
+        std::sort(xg[g].begin(), xg[g].end());
+        std::vector<fwi::Task> kinds[3];
+        for (int i : xg[g])
+            for (int q = 0; q < 3; ++q) kinds[q].insert(kinds[q].end(), kinds_of[i][q].begin(), kinds_of[i][q].end());
+        const size_t total = kinds[0].size() + kinds[1].size() + kinds[2].size();
+        size_t taken[3] = {0, 0, 0};
+        for (size_t n = 0; n < total; ++n) {
+            // issue the kind that is furthest behind its share
+            int best = -1;
+            double lag = -1e300;
+            for (int q = 0; q < 3; ++q) {
+                if (taken[q] >= kinds[q].size()) continue;
+                const double d = (double)(n + 1) * kinds[q].size() / total - taken[q];
+                if (d > lag) { lag = d; best = q; }
+            }
+            gt[g].push_back(kinds[best][taken[best]++]);
+        }
+        per = std::max(per, gt[g].size());
+    }
+    std::vector<fwi::Task> tasks(8 * per);
+    for (auto& t : tasks) { std::memset(&t, 0, sizeof(t)); t.kind = fwi::TASK_IDLE; }
+    for (int g = 0; g < 8; ++g)
+        for (size_t j = 0; j < gt[g].size(); ++j) tasks[g * per + j] = gt[g][j];
+    return tasks;
+}
+
+int build_groups(fwi_ctx* c);
 
 int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     c->release();
@@ -401,52 +468,12 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         s.o_obs = o_obs + s.toff * sizeof(float);
         s.o_adj = o_adj + s.toff * sizeof(float);
     }
-    // XCD groups: block b runs on XCD b % 8 and takes task (b % 8) * per + b / 8,
-    // so group g is a contiguous range of `per` tasks.  Shots go to groups
-    // largest first onto the least-loaded group; inside a group the (slower,
-    // latency-bound) frame tasks come first.
-    std::vector<int> order(nshots);
-    for (int i = 0; i < nshots; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
-    std::vector<std::vector<int>> groups(8);
-    std::vector<long long> load(8, 0);
-    for (int i : order) {
-        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        groups[g].push_back(i);
-        load[g] += work[i];
-    }
-    // Within a group the task kinds are interleaved in proportion (a
-    // latency-bound frame task next to bandwidth-bound interior tasks keeps
-    // every CU busy on both).
-    std::vector<std::vector<fwi::Task>> gt(8);
-    size_t per = 0;
-    for (int g = 0; g < 8; ++g) {
-        std::sort(groups[g].begin(), groups[g].end());
-        std::vector<fwi::Task> kinds[3];
-        for (int i : groups[g]) {
-            kinds[0].insert(kinds[0].end(), ftasks[i].begin(), ftasks[i].end());
-            kinds[1].insert(kinds[1].end(), ytasks[i].begin(), ytasks[i].end());
-            kinds[2].insert(kinds[2].end(), itasks[i].begin(), itasks[i].end());
-        }
-        const size_t total = kinds[0].size() + kinds[1].size() + kinds[2].size();
-        size_t taken[3] = {0, 0, 0};
-        for (size_t n = 0; n < total; ++n) {
-            // issue the kind that is furthest behind its share
-            int best = -1;
-            double lag = -1e300;
-            for (int q = 0; q < 3; ++q) {
-                if (taken[q] >= kinds[q].size()) continue;
-                const double d = (double)(n + 1) * kinds[q].size() / total - taken[q];
-                if (d > lag) { lag = d; best = q; }
-            }
-            gt[g].push_back(kinds[best][taken[best]++]);
-        }
-        per = std::max(per, gt[g].size());
-    }
-    std::vector<fwi::Task> tasks(8 * per);
-    for (auto& t : tasks) { std::memset(&t, 0, sizeof(t)); t.kind = fwi::TASK_IDLE; }
-    for (int g = 0; g < 8; ++g)
-        for (size_t j = 0; j < gt[g].size(); ++j) tasks[g * per + j] = gt[g][j];
+    c->shot_kinds.assign(nshots, {});
+    for (int i = 0; i < nshots; ++i) c->shot_kinds[i] = {ftasks[i], ytasks[i], itasks[i]};
+    c->shot_work = work;
+    std::vector<int> all(nshots);
+    for (int i = 0; i < nshots; ++i) all[i] = i;
+    const std::vector<fwi::Task> tasks = xcd_task_list(c->shot_kinds, work, all);
     c->ntasks = (int)tasks.size();
 
     // misc arena: descriptors, tasks, misfit scratch, gradient
@@ -557,6 +584,62 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     HIPCHK(hipMemset(c->traces.base, 0, c->traces.size));
     HIPCHK(hipMemset(c->grads.base, 0, c->grads.size));
     c->have_shots = true;
+    return build_groups(c);
+}
+
+// Shot groups of the current setting: contiguous shot ranges of near-equal
+// work, each with its own task list.
+constexpr size_t kGroupStateBytes = size_t(192) << 20;
+
+int build_groups(fwi_ctx* c) {
+    const int ns = (int)c->shots.size();
+    c->drop_graphs();
+    c->gtasks.release();
+    c->sgroups.clear();
+    if (ns == 0) return 0;
+    int G = c->want_groups;
+    if (G <= 0) G = (int)((c->state.size + kGroupStateBytes - 1) / kGroupStateBytes);
+    G = std::max(1, std::min(G, ns));
+    long long total = 0;
+    for (long long w : c->shot_work) total += w;
+    std::vector<std::vector<fwi::Task>> lists;
+    long long acc = 0;
+    int s0 = 0;
+    for (int g = 0; g < G; ++g) {
+        // at least one shot; one left for every later group; stop where the
+        // running work would pass this group's share by more than half a shot
+        const long long target = total * (g + 1) / G;
+        int s1 = s0;
+        while (s1 < ns) {
+            const bool first = s1 == s0;
+            const bool reserved = ns - s1 <= G - g - 1;
+            if (!first && (reserved || acc + c->shot_work[s1] / 2 > target)) break;
+            acc += c->shot_work[s1];
+            ++s1;
+        }
+        if (g == G - 1)
+            while (s1 < ns) acc += c->shot_work[s1++];
+        fwi_ctx::ShotGroup grp;
+        grp.s0 = s0;
+        grp.s1 = s1;
+        grp.st0 = c->shots[s0].o_p[0];
+        grp.st1 = s1 < ns ? c->shots[s1].o_p[0] : c->state.size;
+        std::vector<int> ids;
+        for (int i = s0; i < s1; ++i) ids.push_back(i);
+        lists.push_back(xcd_task_list(c->shot_kinds, c->shot_work, ids));
+        c->sgroups.push_back(grp);
+        s0 = s1;
+    }
+    std::vector<size_t> off;
+    for (auto& l : lists) off.push_back(c->gtasks.add(std::max<size_t>(1, l.size()) * sizeof(fwi::Task)));
+    CHK(c->gtasks.alloc());
+    for (size_t g = 0; g < lists.size(); ++g) {
+        c->sgroups[g].d_tasks = c->gtasks.at<fwi::Task>(off[g]);
+        c->sgroups[g].ntasks = (int)lists[g].size();
+        if (!lists[g].empty())
+            HIPCHK(hipMemcpy(c->sgroups[g].d_tasks, lists[g].data(), lists[g].size() * sizeof(fwi::Task),
+                             hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -684,14 +767,17 @@ int in_graph(fwi_ctx* c, const std::string& key, F&& issue) {
 // (mode 1, receivers inject): the step k with k % ss == 0 correlates with
 // slot m = nt_store - k/ss, read from buffer slot m - off (slot 0: the
 // zero slot).
-int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int mode, bool record, int k0, int k1, int lo, int hi,
-                int off, bool hist = false) {
+// grp: the shot group stepped (-1: every shot in one task list).
+int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int grp, int mode, bool record, int k0, int k1, int lo,
+                int hi, int off, bool hist = false) {
     const fwi_config& cf = c->cfg;
     fwi::StepParams prm;
     std::memset(&prm, 0, sizeof(prm));
     prm.shots = desc;
-    prm.tasks = c->d_tasks;
-    prm.ntasks = c->ntasks;
+    prm.tasks = grp < 0 ? c->d_tasks : c->sgroups[grp].d_tasks;
+    prm.ntasks = grp < 0 ? c->ntasks : c->sgroups[grp].ntasks;
+    const int q0 = grp < 0 ? 0 : c->sgroups[grp].s0;
+    const int nq = grp < 0 ? (int)c->shots.size() : c->sgroups[grp].s1 - c->sgroups[grp].s0;
     prm.rows = c->d_rows;
     prm.ents = c->d_ents;
     prm.Pm = c->Pm;
@@ -733,8 +819,8 @@ int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int mode, bool record, int
         if (hist)
             HIPCHK(fwi::launch_hist_store(desc, c->d_exact, (int)c->shots.size(), c->exact_max_stride, k, c->stream));
         if (prm.snap_op == 1 && q16)
-            HIPCHK(fwi::launch_quantize(c->d_quant, (int)c->shots.size(), c->max_slot_elems, prm.slot,
-                                        cf.snap_kind == fwi::SNAP_BF16, c->stream));
+            HIPCHK(fwi::launch_quantize(c->d_quant + q0, nq, c->max_slot_elems, prm.slot, cf.snap_kind == fwi::SNAP_BF16,
+                                        c->stream));
     }
     return 0;
 }
@@ -762,7 +848,8 @@ int timing_end(fwi_ctx* c, hipEvent_t ev0) {
 }
 
 // Forward sweep k = 2..nt: receivers sampled; with store, snapshots (and the
-// checkpoints of the checkpointed schedule).
+// checkpoints of the checkpointed schedule).  Shot groups run one after the
+// other, each through every step.
 int forward_sweep(fwi_ctx* c, bool store, bool hist = false) {
     const fwi_config& cf = c->cfg;
     const int nt = cf.nt, ss = cf.store_snap;
@@ -773,31 +860,38 @@ int forward_sweep(fwi_ctx* c, bool store, bool hist = false) {
     hipEvent_t ev0;
     CHK(timing_begin(c, &ev0));
     if (hist) {
-        CHK(issue_steps(c, c->d_desc, 0, true, 2, nt, 1, 0, 0, true));
-    } else if (!store) {
-        CHK(in_graph(c, "forward", [&] { return issue_steps(c, c->d_desc, 0, true, 2, nt, 1, 0, 0); }));
-    } else if (c->seg == 0) {
-        CHK(in_graph(c, "forward_store", [&] {
-            CHK(reset_snapmax(c));
-            return issue_steps(c, c->d_desc, 0, true, 2, nt, 1, c->nslots, 0);
-        }));
+        CHK(issue_steps(c, c->d_desc, -1, 0, true, 2, nt, 1, 0, 0, true));
     } else {
-        const int S = c->seg, nseg = num_segments(c, S);
-        CHK(in_graph(c, "forward_checkpoints", [&] {
-            CHK(reset_snapmax(c));
-            int k = 2;
-            for (int j = 1; j < nseg; ++j) {
-                const int kc = j * S * ss;
-                CHK(issue_steps(c, c->d_desc, 0, true, k, kc, 1, 0, 0));
-                HIPCHK(fwi::launch_copy16(c->ckpt.base + (size_t)(j - 1) * c->state.size, c->state.base,
-                                          c->state.size / 16, c->stream));
-                k = kc + 1;
+        for (int g = 0; g < (int)c->sgroups.size(); ++g) {
+            const std::string gs = std::to_string(g);
+            const fwi_ctx::ShotGroup& G = c->sgroups[g];
+            if (!store) {
+                CHK(in_graph(c, "forward_" + gs, [&] { return issue_steps(c, c->d_desc, g, 0, true, 2, nt, 1, 0, 0); }));
+            } else if (c->seg == 0) {
+                CHK(in_graph(c, "forward_store_" + gs, [&] {
+                    CHK(reset_snapmax(c));
+                    return issue_steps(c, c->d_desc, g, 0, true, 2, nt, 1, c->nslots, 0);
+                }));
+            } else {
+                const int S = c->seg, nseg = num_segments(c, S);
+                CHK(in_graph(c, "forward_checkpoints_" + gs, [&] {
+                    CHK(reset_snapmax(c));
+                    int k = 2;
+                    for (int j = 1; j < nseg; ++j) {
+                        const int kc = j * S * ss;
+                        CHK(issue_steps(c, c->d_desc, g, 0, true, k, kc, 1, 0, 0));
+                        HIPCHK(fwi::launch_copy16(c->ckpt.base + (size_t)(j - 1) * c->state.size + G.st0,
+                                                  c->state.base + G.st0, (G.st1 - G.st0) / 16, c->stream));
+                        k = kc + 1;
+                    }
+                    if (nseg == 0) return issue_steps(c, c->d_desc, g, 0, true, k, nt, 1, 0, 0);
+                    const int j = nseg - 1;
+                    return issue_steps(c, c->d_desc, g, 0, true, k, nt, j * S + 1, std::min((j + 1) * S, c->nt_store - 1),
+                                       j * S);
+                }));
             }
-            if (nseg == 0) return issue_steps(c, c->d_desc, 0, true, k, nt, 1, 0, 0);
-            const int j = nseg - 1;
-            return issue_steps(c, c->d_desc, 0, true, k, nt, j * S + 1, std::min((j + 1) * S, c->nt_store - 1), j * S);
-        }));
-        c->last_segment_resident = true;
+        }
+        if (store && c->seg > 0) c->last_segment_resident = true;
     }
     CHK(timing_end(c, ev0));
     // Rec(nt,:) = Rec(nt-1,:)  (solver_acoustic_SS.f90:588)
@@ -814,9 +908,9 @@ int forward_sweep(fwi_ctx* c, bool store, bool hist = false) {
 }
 
 // Adjoint sweep k = 2..nt with the imaging condition (the per-shot kernels
-// accumulate; the caller zeroes them).  With checkpoints, each earlier
-// segment is recomputed from its checkpoint just before the adjoint reaches
-// the steps that read it.
+// accumulate; the caller zeroes them), shot group by shot group.  With
+// checkpoints, each earlier segment of a group is recomputed from its
+// checkpoint just before the adjoint reaches the steps that read it.
 int adjoint_sweep(fwi_ctx* c) {
     const fwi_config& cf = c->cfg;
     const int nt = cf.nt, ss = cf.store_snap;
@@ -825,31 +919,42 @@ int adjoint_sweep(fwi_ctx* c) {
     HIPCHK(hipMemsetAsync(ast.base, 0, ast.size, c->stream));
     hipEvent_t ev0;
     CHK(timing_begin(c, &ev0));
-    if (c->seg == 0) {
-        CHK(in_graph(c, "adjoint", [&] { return issue_steps(c, c->d_desc_adj, 1, false, 2, nt, 0, 0, 0); }));
-    } else {
+    const bool resident = c->last_segment_resident;      // the forward left every group's last segment
+    bool recomputed = false;
+    for (int g = 0; g < (int)c->sgroups.size(); ++g) {
+        const std::string gs = std::to_string(g);
+        const fwi_ctx::ShotGroup& G = c->sgroups[g];
+        if (c->seg == 0) {
+            CHK(in_graph(c, "adjoint_" + gs, [&] { return issue_steps(c, c->d_desc_adj, g, 1, false, 2, nt, 0, 0, 0); }));
+            continue;
+        }
         const int S = c->seg, nseg = num_segments(c, S);
         int k = 2;
         for (int j = nseg - 1; j >= 0; --j) {
             const int lo = j * S + 1, hi = std::min((j + 1) * S, c->nt_store - 1);
-            if (j < nseg - 1 || !c->last_segment_resident) {
-                if (j == 0) HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
-                CHK(in_graph(c, "recompute_" + std::to_string(j), [&] {
+            const std::string js = gs + "_" + std::to_string(j);
+            if (j < nseg - 1 || !resident) {
+                if (j == 0) HIPCHK(hipMemsetAsync(c->state.base + G.st0, 0, G.st1 - G.st0, c->stream));
+                CHK(in_graph(c, "recompute_" + js, [&] {
                     if (j > 0)
-                        HIPCHK(fwi::launch_copy16(c->state.base, c->ckpt.base + (size_t)(j - 1) * c->state.size,
-                                                  c->state.size / 16, c->stream));
+                        HIPCHK(fwi::launch_copy16(c->state.base + G.st0,
+                                                  c->ckpt.base + (size_t)(j - 1) * c->state.size + G.st0,
+                                                  (G.st1 - G.st0) / 16, c->stream));
                     CHK(reset_snapmax(c));
-                    return issue_steps(c, c->d_desc, 0, false, j * S * ss + 1, hi * ss, lo, hi, j * S);
+                    return issue_steps(c, c->d_desc, g, 0, false, j * S * ss + 1, hi * ss, lo, hi, j * S);
                 }));
-                c->last_segment_resident = false;
+                recomputed = true;
             }
             const int kend = (c->nt_store - lo) * ss;
-            CHK(in_graph(c, "adjoint_" + std::to_string(j),
-                         [&] { return issue_steps(c, c->d_desc_adj, 1, false, k, kend, 0, 0, j * S); }));
+            CHK(in_graph(c, "adjoint_" + js,
+                         [&] { return issue_steps(c, c->d_desc_adj, g, 1, false, k, kend, 0, 0, j * S); }));
             k = kend + 1;
         }
-        if (k <= nt) CHK(in_graph(c, "adjoint_end", [&] { return issue_steps(c, c->d_desc_adj, 1, false, k, nt, 0, 0, 0); }));
+        if (k <= nt)
+            CHK(in_graph(c, "adjoint_end_" + gs,
+                         [&] { return issue_steps(c, c->d_desc_adj, g, 1, false, k, nt, 0, 0, 0); }));
     }
+    if (recomputed) c->last_segment_resident = false;
     CHK(timing_end(c, ev0));
     return 0;
 }
@@ -1332,33 +1437,46 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
                            s.ry.data()};
     }
     const int k1 = std::min(c->cfg.nt, steps + 1);
-    const int candidates[] = {4, 6, 8, 12, 16, 24, 32};
-    int best = -1;
+    const int row_cands[] = {4, 6, 8, 12, 16, 24, 32};
+    const int want = c->want_groups;
+    // group counts: the setting itself when fixed, else 1 .. 4
+    std::vector<int> group_cands;
+    if (want > 0) group_cands.push_back(want);
+    else for (int g = 1; g <= std::min(4, (int)defs.size()); ++g) group_cands.push_back(g);
+    int best = -1, best_g = 0;
     double best_us = 0.0;
     std::vector<float> vp((size_t)c->cfg.ny * c->cfg.nx, 2000.0f);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    for (int r : candidates) {
+    for (int r : row_cands) {
         if (r > fwi::MAX_TASK_ROWS) continue;
         c->rows = r;
+        c->want_groups = 0;
         CHK(setup_shots(c, (int)defs.size(), defs.data()));
         CHK(fwi_set_model(c, vp.data()));
-        HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
-        CHK(issue_steps(c, c->d_desc, 0, false, 2, std::min(k1, 4), 1, 0, 0));      // warm-up
-        HIPCHK(hipEventRecord(e0, c->stream));
-        CHK(issue_steps(c, c->d_desc, 0, false, 2, k1, 1, 0, 0));
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        const double us = 1000.0 * ms / (k1 - 1);
-        if (best < 0 || us < best_us) { best = r; best_us = us; }
+        for (int G : group_cands) {
+            c->want_groups = G;
+            CHK(build_groups(c));
+            HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+            for (int g = 0; g < (int)c->sgroups.size(); ++g)                                   // warm-up
+                CHK(issue_steps(c, c->d_desc, g, 0, false, 2, std::min(k1, 4), 1, 0, 0));
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int g = 0; g < (int)c->sgroups.size(); ++g)
+                CHK(issue_steps(c, c->d_desc, g, 0, false, 2, k1, 1, 0, 0));
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1000.0 * ms / (k1 - 1);
+            if (best < 0 || us < best_us) { best = r; best_g = want > 0 ? want : G; best_us = us; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->rows = best;
     c->cfg.rows_per_task = best;
+    c->want_groups = best_g;
     CHK(setup_shots(c, (int)defs.size(), defs.data()));
     if (best_rows) *best_rows = best;
     if (best_step_us) *best_step_us = best_us;
@@ -1366,6 +1484,17 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
 }
 
 int fwi_get_rows_per_task(fwi_ctx* c) { return c ? c->rows : -1; }
+
+int fwi_set_shot_groups(fwi_ctx* c, int groups) {
+    if (!c) return fail("null context");
+    if (groups < 0) return fail("fwi_set_shot_groups: groups must be >= 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->want_groups = groups;
+    if (c->have_shots) CHK(build_groups(c));
+    return 0;
+}
+
+int fwi_get_shot_groups(fwi_ctx* c) { return c ? (int)c->sgroups.size() : -1; }
 
 // ---------------------------------------------------------------------------
 // Trace conditioning (time_filter.f90, interpolation_akima.f90)

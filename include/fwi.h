@@ -245,14 +245,25 @@ int fwi_plan_schedule(fwi_ctx* ctx, long long budget_bytes, int* segment_slots);
 long long fwi_schedule_bytes(fwi_ctx* ctx, int segment_slots);
 
 /* Autotuner (no counterpart in the reference): times `steps` forward steps of
- * the context's own shots (all shots in every launch) for each candidate
- * number of rows per work unit, keeps the fastest and rebuilds the task
- * list.  Call it right after fwi_set_shots: it re-creates the shot state,
+ * the context's own shots for each candidate number of rows per work unit
+ * (and each shot-group count 1..4 unless fwi_set_shot_groups fixed one),
+ * keeps the fastest and rebuilds the task list.  Call it right after fwi_set_shots: it re-creates the shot state,
  * so the model, sources and data are uploaded afterwards.  Returns the rows
  * chosen and the measured time per step. */
 int fwi_autotune(fwi_ctx* ctx, int steps, int* best_rows, double* best_step_us);
 /* Rows per work unit of the current task list. */
 int fwi_get_rows_per_task(fwi_ctx* ctx);
+
+/* Shot groups (no counterpart in the reference, whose MPI ranks each step
+ * their own shots to the end, calculate_misfit_gradient.f90:73-360): the
+ * sweeps advance one group of contiguous shots through all its time steps
+ * before the next group, so the group's propagation state stays in the
+ * GPU's last-level cache between steps.  groups = 0 (default) sizes the
+ * groups to 192 MiB of state each; fwi_autotune tries 1 to 4 groups when
+ * the setting is 0.  Results are bit-identical for any grouping.
+ * fwi_get_shot_groups returns the number of groups in use. */
+int fwi_set_shot_groups(fwi_ctx* ctx, int groups);
+int fwi_get_shot_groups(fwi_ctx* ctx);
 
 /* Time-step sweeps are captured as HIP graphs on first use and replayed
  * (enable = 1, default); enable = 0 launches every step directly. */

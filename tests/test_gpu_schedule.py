@@ -152,3 +152,25 @@ def test_16bit_snapshot_error_vs_fp64_reference():
         e, cos = _rel(out[kind], A8.kernel), _cos(out[kind], A8.kernel)
         print(f"\n{name} snapshots vs FP64 reference gradient: rel-L2 {e:.2e}, cosine {cos:.9f}")
         assert e < BOUND.get(kind, 1e-4) and cos >= 0.9999
+
+
+@pytest.mark.parametrize("groups,seg,snap_kind", [(2, 0, fwi_amd.SNAP_F32), (3, 7, fwi_amd.SNAP_F32),
+                                                  (2, 0, fwi_amd.SNAP_BF16), (4, 20, fwi_amd.SNAP_F16)])
+def test_shot_groups_are_bit_identical(groups, seg, snap_kind):
+    """Shot groups (each group swept through every step before the next, with
+    its own checkpoints) give the single-group traces, misfit and gradient bit
+    for bit."""
+    out = []
+    for g in (1, groups):
+        full, ctx = _marine(nshots=5, snap_kind=snap_kind)
+        ctx.set_shot_groups(g)
+        assert ctx.shot_groups() == g
+        if seg:
+            ctx.set_schedule(seg)
+        m, grad, ker = _grad(ctx, full)
+        out.append((m, grad, ker, ctx.synthetic()))
+    (m0, g0, k0, s0), (m1, g1, k1, s1) = out
+    assert np.abs(g0).max() > 0
+    assert m0 == m1 and np.array_equal(g0, g1)
+    for a, b in zip(k0 + s0, k1 + s1):
+        assert np.array_equal(a, b)
