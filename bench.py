@@ -10,8 +10,10 @@ shot sum with the water mask -- all on the GPU, inputs resident in HBM.
 
 Prints one JSON line.  `value` is credited cell-updates per second
 (padded window cells x (nt-1) steps x 2 sweeps, summed over shots).
-`roofline` is the fused step kernel's algorithmic HBM bytes per launch
-divided by its average launch duration (HIP events on the context's stream).
+`roofline` is the fused step kernel's algorithmic HBM bytes per time step
+of all shots divided by the time of that step (HIP events on the context's
+stream around the sweeps; a step of all shots is one launch per shot group,
+groups being swept one after the other).
 `cpu_baseline` times the reference's own Fortran solver (compiled from its
 sources into oracle/_ref) on a bounded sample of the same workload, one
 process per core as `mpirun -np C` over shots would run it.
@@ -166,9 +168,10 @@ def main():
     tuned = None
     if not a.rows:
         rows, us = ctx.autotune(a.tune_steps)
-        tuned = {"rows_per_task": rows, "step_us": us, "steps_timed": a.tune_steps}
-        print(f"[bench] autotune: {rows} rows per work unit ({us:.1f} us per forward step)", file=sys.stderr,
-              flush=True)
+        tuned = {"rows_per_task": rows, "shot_groups": ctx.shot_groups(), "step_us": us,
+                 "steps_timed": a.tune_steps}
+        print(f"[bench] autotune: {rows} rows per work unit, {ctx.shot_groups()} shot groups ({us:.1f} us per "
+              f"forward step)", file=sys.stderr, flush=True)
     ctx.set_sources(acq.wavelets)
     ctx.set_model(acq.vp_true)
     ctx.forward(store_snapshots=False)           # observed data from the true model
@@ -199,10 +202,12 @@ def main():
     cells = acq.cell_updates() * a.steps
     gcell = cells / wall / 1e9
     bytes_per_grad = algorithmic_bytes(acq, snap_bytes)
-    launches_per_grad = 2 * (acq.nt - 1)
-    avg_launch_s = step_ms / 1e3 / max(nlaunch, 1)
-    bytes_per_launch = bytes_per_grad / launches_per_grad
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    # one time step of every shot is one step_kernel launch per shot group
+    groups = ctx.shot_groups()
+    steps_per_grad = 2 * (acq.nt - 1)
+    avg_step_s = step_ms / 1e3 / max(nlaunch, 1)
+    bytes_per_step = bytes_per_grad / steps_per_grad
+    achieved = bytes_per_step / avg_step_s / 1e9
     out = {
         "metric": METRIC,
         "value": gcell,
@@ -224,6 +229,7 @@ def main():
             "nt": acq.nt,
             "snapshots": a.snap,
             "rows_per_task": ctx.rows_per_task(),
+            "shot_groups": groups,
             "autotuned": tuned,
         },
         "seconds_per_gradient": wall / a.steps,
@@ -236,9 +242,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": avg_launch_s * 1e6,
-            "launches": nlaunch,
+            "algorithmic_bytes_per_step": bytes_per_step,
+            "avg_step_us": avg_step_s * 1e6,
+            "launches_per_step": groups,
+            "algorithmic_bytes_per_launch": bytes_per_step / groups,
+            "avg_launch_us": avg_step_s * 1e6 / groups,
+            "steps_timed": nlaunch,
             "frac_of_wall_in_step_kernel": (step_ms / 1e3) / wall,
         },
         "cpu_baseline": cpu,
