@@ -158,5 +158,20 @@ module fwi_hip
        type(c_ptr), value :: ctx
        real(c_float), intent(out) :: vp(*)
      end function fwi_get_model
+
+     ! tuning: work-unit height and shot groups (call right after fwi_set_shots)
+     integer(c_int) function fwi_autotune(ctx, steps, best_rows, best_step_us) bind(C, name="fwi_autotune")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: steps
+       integer(c_int), intent(out) :: best_rows
+       real(c_double), intent(out) :: best_step_us
+     end function fwi_autotune
+
+     integer(c_int) function fwi_set_shot_groups(ctx, groups) bind(C, name="fwi_set_shot_groups")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: groups
+     end function fwi_set_shot_groups
   end interface
 end module fwi_hip
