@@ -98,22 +98,22 @@ def single_shot_400x150(free_surface: int = 0) -> Acquisition:
     return acq
 
 
-def marine_64shot_2000x400(seed: int = 0, nshots: int = 64, nt: int = 5000) -> Acquisition:
-    """Batched-shot workload: nx=2000, nz=400, h=25 m, dt=2 ms, nt=5000, 64 shots.
+def marine_layered(name: str, ny: int, nx: int, h: float, dt: float, nt: int, nshots: int, wmin: int, wmax: int,
+                   seed: int = 0) -> Acquisition:
+    """Layered marine model on an (ny, nx) grid of spacing h with OBS-style shots.
 
-    Seed-0 layered marine model: water (1500 m/s) over a smooth random
-    seafloor at 1.5-3 km, then 5 layers at 2000, 2800, 3800, 5000, 6500 m/s
-    with smooth random interfaces and smooth +-5 % perturbations (Courant
-    number <= 6825*0.002/25 = 0.546).  Each shot: Ricker with f0 ~ U(6,10)
-    Hz at 10 m depth, aperture window W ~ U[600,1600] cells at a random
-    position, source at a random point of the central half of its window,
-    one receiver per window column at the seafloor cell (OBS style)."""
-    ny, nx, h, dt = 400, 2000, 25.0, 0.002
+    Water (1500 m/s) over a smooth random seafloor at 1.5-3 km, then 5 layers
+    at 2000, 2800, 3800, 5000, 6500 m/s with smooth random interfaces and
+    smooth +-5 % perturbations; smoothing lengths are fixed in metres, so the
+    model is the same geology on any grid.  Each shot: Ricker with
+    f0 ~ U(6,10) Hz at 10 m depth, aperture window W ~ U[wmin, wmax] cells at a
+    random position, source at a random point of the central half of its
+    window, one receiver per window column at the seafloor cell."""
     rng = np.random.default_rng(seed)
-    xs = np.arange(nx)
+    k = 25.0 / h                                              # smoothing lengths given in 25 m cells
 
     def smooth_curve(lo, hi, corr):
-        c = _smooth(rng.standard_normal(nx), corr)
+        c = _smooth(rng.standard_normal(nx), corr * k)
         c = (c - c.min()) / max(c.max() - c.min(), 1e-12)
         return lo + (hi - lo) * c
 
@@ -130,19 +130,19 @@ def marine_64shot_2000x400(seed: int = 0, nshots: int = 64, nt: int = 5000) -> A
     for li, v in enumerate(vel):
         top = bounds[li][None, :]
         vp = np.where(z >= top, v, vp)
-    pert = _smooth(rng.standard_normal((ny, nx)), 8.0)
+    pert = _smooth(rng.standard_normal((ny, nx)), 8.0 * k)
     pert = 0.05 * pert / np.abs(pert).max()
     sub = z >= seafloor[None, :]
     vp = np.where(sub, vp * (1.0 + pert), vp)
     vtrue = vp.astype(np.float32)
-    vinit = _smooth(vp, 20.0)
+    vinit = _smooth(vp, 20.0 * k)
     vinit = np.where(sub, vinit, 1500.0).astype(np.float32)
     wr = np.array([grid_index(seafloor[i], h) for i in range(nx)], np.int32)
 
     shots, wavelets = [], []
     sy = grid_index(10.0, h)
     for _ in range(nshots):
-        w = int(rng.integers(600, 1601))
+        w = int(rng.integers(wmin, wmax + 1))
         x0 = int(rng.integers(0, nx - w + 1))
         f0 = float(rng.uniform(6.0, 10.0))
         sx = int(round(rng.uniform(0.25, 0.75) * w)) + 1
@@ -150,8 +150,56 @@ def marine_64shot_2000x400(seed: int = 0, nshots: int = 64, nt: int = 5000) -> A
         ry = wr[x0:x0 + w]
         shots.append(dict(x0=x0, w=w, src_x=[sx], src_y=[sy], rec_x=rx.tolist(), rec_y=ry.tolist(), f0=f0))
         wavelets.append(ricker(f0, nt, dt)[:, None])
-    acq = Acquisition("marine_64shot_2000x400", ny, nx, h, dt, nt, vtrue, vinit, wr, shots, wavelets, free_surface=0)
+    acq = Acquisition(name, ny, nx, h, dt, nt, vtrue, vinit, wr, shots, wavelets, free_surface=0)
     acq.notes = {"seed": seed, "vmax": float(vtrue.max()), "courant": float(vtrue.max()) * dt / h}
+    return acq
+
+
+def marine_64shot_2000x400(seed: int = 0, nshots: int = 64, nt: int = 5000) -> Acquisition:
+    """Batched-shot workload: nx=2000, nz=400, h=25 m, dt=2 ms, nt=5000, 64 shots
+    (marine_layered; Courant number <= 6825*0.002/25 = 0.546), windows of
+    600-1600 cells."""
+    return marine_layered("marine_64shot_2000x400", 400, 2000, 25.0, 0.002, nt, nshots, 600, 1600, seed)
+
+
+def checkpoint_6000x1000(seed: int = 0, nshots: int = 16, nt: int = 15000) -> Acquisition:
+    """Checkpointed-adjoint workload: nx=6000, nz=1000, h=10 m, dt=0.8 ms,
+    nt=15000 (Courant number 0.546 as above).  The model is not specified
+    beyond the grid; recorded choice: the layered marine model on this grid,
+    windows of 1500-3000 cells (15-30 km)."""
+    return marine_layered("checkpoint_6000x1000", 1000, 6000, 10.0, 0.0008, nt, nshots, 1500, 3000, seed)
+
+
+def precision_4000x800(nshots: int = 32, nt: int = 10000, window: int = 1200) -> Acquisition:
+    """Precision workload: the single-shot model (single_shot_400x150) on a finer,
+    wider grid -- nx=4000, nz=800, h=12.5 m, dt=1 ms, nt=10000.  Water to 2 km,
+    then 1800 -> 4500 m/s to the bottom, +8 % Gaussian anomaly (sigma 500 m)
+    at the model centre; initial model smoothed with sigma 500 m, water kept.
+    Recorded choices for the 32 shots: evenly spaced windows of `window`
+    columns (15 km), an 8 Hz Ricker at 10 m depth in the window centre and
+    one receiver per window column at 10 m depth."""
+    ny, nx, h, dt = 800, 4000, 12.5, 0.001
+    z = np.arange(ny) * h
+    x = np.arange(nx) * h
+    zb = z[-1]
+    v1 = np.where(z < 2000.0, 1500.0, 1800.0 + (z - 2000.0) / (zb - 2000.0) * 2700.0)
+    vp = np.repeat(v1[:, None], nx, axis=1)
+    xc, zc = x[-1] / 2, z[-1] / 2
+    r2 = (x[None, :] - xc) ** 2 + (z[:, None] - zc) ** 2
+    vt = vp * (1.0 + 0.08 * np.exp(-r2 / (2 * 500.0 ** 2)))
+    water = z < 2000.0
+    vi = _smooth(vt, 500.0 / h)
+    vi[water, :] = 1500.0
+    wr = np.full(nx, grid_index(2000.0, h), np.int32)
+    sy = grid_index(10.0, h)
+    shots, wavelets = [], []
+    starts = np.linspace(0, nx - window, nshots).round().astype(int) if nshots > 1 else [0]
+    for x0 in starts:
+        shots.append(dict(x0=int(x0), w=window, src_x=[window // 2 + 1], src_y=[sy],
+                          rec_x=list(range(1, window + 1)), rec_y=[sy] * window, f0=8.0))
+        wavelets.append(ricker(8.0, nt, dt)[:, None])
+    acq = Acquisition("precision_4000x800", ny, nx, h, dt, nt, vt.astype(np.float32), vi.astype(np.float32), wr,
+                      shots, wavelets, free_surface=0, store_snap=4)
     return acq
 
 

@@ -1,0 +1,148 @@
+"""Parity at the full sizes of the benchmark configurations, through properties
+that hold at any size (the oracle itself only finishes small grids quickly).
+
+* Checkpointed adjoint, nx=6000 nz=1000 h=10 m dt=0.8 ms nt=15000: one shot's
+  checkpointed gradient equals its stored-snapshot gradient bit for bit, and a
+  16-shot batch planned under a 64 GB snapshot budget gives the same gradient
+  with half the checkpoint spacing.
+* 16-bit snapshots, nx=4000 nz=800 h=12.5 m dt=1 ms nt=10000, snapshots every
+  4 steps: FP16 within 1e-3 and BF16 within 2e-3 relative L2 (cosine >= 0.9999)
+  of the FP32-snapshot gradient; the distances are printed.
+* Marine 64-shot gradient, nx=2000 nz=400 nt=5000: the shot grouping of the
+  sweeps does not change a bit, and the exact discrete adjoint passes the
+  dot-product test (normal-operator form) on a full-size shot window.
+"""
+import gc
+
+import numpy as np
+import pytest
+
+import fwi_amd
+from fwi_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _cos(a, b):
+    a, b = a.ravel().astype(np.float64), b.ravel().astype(np.float64)
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+
+
+def _context(acq, shots, wav, snap_kind=fwi_amd.SNAP_F32):
+    ctx = fwi_amd.Context(acq.ny, acq.nx, np.float32(acq.h), np.float32(acq.dt), acq.nt, acq.store_snap,
+                          dpml=acq.dpml, free_surface=acq.free_surface, snap_kind=snap_kind)
+    ctx.set_shots(shots)
+    ctx.set_sources(wav)
+    ctx.set_model(acq.vp_true)
+    ctx.forward(store_snapshots=False)
+    ctx.observed_from_synthetic()
+    ctx.set_model(acq.vp_init)
+    return ctx
+
+
+def _gradient(ctx, acq, nshots):
+    m = ctx.gradient(1.0 / nshots, acq.water_rows)
+    g = ctx.get_gradient()
+    ctx.close()
+    gc.collect()
+    return m, g
+
+
+def test_checkpoint_6000x1000_single_shot_matches_stored():
+    acq = synth.checkpoint_6000x1000(nshots=1)
+    ctx = _context(acq, acq.shots, acq.wavelets)
+    stored = ctx.schedule_bytes(0)
+    m0, g0 = _gradient(ctx, acq, 1)
+    ctx = _context(acq, acq.shots, acq.wavelets)
+    seg = ctx.plan_schedule(stored // 4)
+    assert seg > 0
+    ctx.set_schedule(seg)
+    assert ctx.schedule_bytes(seg) <= stored // 4
+    m1, g1 = _gradient(ctx, acq, 1)
+    print(f"\n6000x1000 nt=15000 shot (W={acq.shots[0]['w']}): stored snapshots {stored / 2**30:.1f} GiB, "
+          f"checkpoints every {seg} snapshots in {stored / 4 / 2**30:.1f} GiB")
+    assert np.abs(g0).max() > 0 and np.isfinite(g0).all()
+    assert m0 == m1 and np.array_equal(g0, g1)
+
+
+def test_checkpoint_6000x1000_16_shots_under_64GB():
+    acq = synth.checkpoint_6000x1000(nshots=16)
+    budget = 64 << 30
+    out = []
+    seg = None
+    for run in range(2):
+        ctx = _context(acq, acq.shots, acq.wavelets)
+        if run == 0:
+            seg = ctx.plan_schedule(budget)          # smallest footprint that fits
+            assert seg > 1, "16 shots of stored FP32 snapshots fit in 64 GB?"
+        else:
+            seg = seg // 2                            # other checkpoint spacing
+            ctx.set_schedule(seg)
+        need = ctx.schedule_bytes(seg)
+        print(f"\n16 shots, 6000x1000 nt=15000: checkpoints every {seg} snapshots, {need / 2**30:.1f} GiB of "
+              f"snapshots and checkpoints (stored: {ctx.schedule_bytes(0) / 2**30:.0f} GiB)")
+        assert need <= budget
+        out.append(_gradient(ctx, acq, 16))
+    (m0, g0), (m1, g1) = out
+    assert np.abs(g0).max() > 0 and np.isfinite(g0).all()
+    assert m0 == m1 and np.array_equal(g0, g1)
+
+
+def test_precision_4000x800_16bit_snapshots():
+    acq = synth.precision_4000x800(nshots=2)
+    grads = {}
+    for kind in (fwi_amd.SNAP_F32, fwi_amd.SNAP_F16, fwi_amd.SNAP_BF16):
+        ctx = _context(acq, acq.shots, acq.wavelets, snap_kind=kind)
+        grads[kind] = _gradient(ctx, acq, 2)[1]
+    g32 = grads[fwi_amd.SNAP_F32]
+    assert np.abs(g32).max() > 0 and np.isfinite(g32).all()
+    for kind, bound in ((fwi_amd.SNAP_F16, 1e-3), (fwi_amd.SNAP_BF16, 2e-3)):
+        r, c = _rel(grads[kind], g32), _cos(grads[kind], g32)
+        print(f"\n4000x800 nt=10000, snapshots every 4 steps, kind {kind}: rel-L2 {r:.2e} vs FP32 snapshots, "
+              f"cosine {c:.8f}")
+        assert r <= bound and c >= 0.9999
+
+
+def test_marine_64_shot_groups_full_size():
+    acq = synth.marine_64shot_2000x400()
+    res = []
+    for groups in (1, 2):
+        ctx = _context(acq, acq.shots, acq.wavelets)
+        ctx.set_shot_groups(groups)
+        assert ctx.shot_groups() == groups
+        res.append(_gradient(ctx, acq, len(acq.shots)))
+    (m0, g0), (m1, g1) = res
+    assert np.abs(g0).max() > 0
+    assert m0 == m1 and np.array_equal(g0, g1)
+
+
+def test_marine_full_window_exact_dot_product():
+    """Dot-product test in its normal-operator form, <F s, F s> = <s, F^T (F s)>,
+    with the shot's Ricker wavelet as s (the FP32 oracle gives 2.8e-6 on this
+    window, FP64 6e-15).  White noise is no test input for an FP32 pair this
+    long: injecting N(0,1) samples every step, the FP32 forward energy drifts
+    1.1 % from FP64 over the 5000 steps and the identity holds to 8.2e-3 only
+    -- on this GPU and, bit for bit, in the FP32 oracle (FP64: 2.5e-13)."""
+    acq = synth.marine_64shot_2000x400()
+    i = int(np.argmax([s["w"] for s in acq.shots]))
+    shot = acq.shots[i]
+    s = acq.wavelets[i]
+    ctx = fwi_amd.Context(acq.ny, acq.nx, np.float32(acq.h), np.float32(acq.dt), acq.nt, 10, dpml=acq.dpml,
+                          free_surface=0)
+    ctx.set_shots([shot])
+    ctx.set_sources([s])
+    ctx.set_model(acq.vp_init)
+    ctx.forward(False)
+    Fs = ctx.synthetic()[0]
+    FtFs = ctx.exact_transpose([Fs])[0]
+    ctx.close()
+    lhs = float((Fs.astype(np.float64) ** 2).sum())
+    rhs = float((s.astype(np.float64) * FtFs).sum())
+    print(f"\nexact dot-product test, W={shot['w']} nt={acq.nt}: <Fs,Fs>={lhs:.9e} <s,F^T Fs>={rhs:.9e} "
+          f"rel {abs(lhs - rhs) / abs(lhs):.1e}")
+    assert lhs > 0
+    assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
