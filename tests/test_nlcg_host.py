@@ -61,3 +61,14 @@ def test_reference_beta_matches_fp64_formula(variant):
     beta64 = float((g64 * (g64 - go64)).sum() / (go64 * go64).sum())
     tol = 1e-13 if dt == np.float64 else 2e-6
     assert b.value == pytest.approx(beta64, rel=tol)
+
+
+def test_get_f1f2_bands():
+    """Band edges of the frequency loop (auxiliar.f90:223-261): growing and sliding bands."""
+    from fwi_amd.multiscale import get_f1f2
+    assert [get_f1f2(i, 3.0, 2.0, 4, 1) for i in (2, 3, 4)] == [(3.0, 5.0), (3.0, 7.0), (3.0, 9.0)]
+    assert [get_f1f2(i, 3.0, 2.0, 4, 2) for i in (2, 3, 4)] == [(3.0, 5.0), (5.0, 7.0), (7.0, 9.0)]
+    with pytest.raises(ValueError):
+        get_f1f2(2, 3.0, 2.0, 4, 3)
+    with pytest.raises(ValueError):
+        get_f1f2(1, 3.0, 2.0, 4, 1)
