@@ -1439,10 +1439,12 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
     const int k1 = std::min(c->cfg.nt, steps + 1);
     const int row_cands[] = {4, 6, 8, 12, 16, 24, 32};
     const int want = c->want_groups;
-    // group counts: the setting itself when fixed, else 1 .. 4
+    // group counts: the setting itself when fixed, else 1..4 and the
+    // cache-sized count with its neighbours (known after the first set-up)
     std::vector<int> group_cands;
     if (want > 0) group_cands.push_back(want);
     else for (int g = 1; g <= std::min(4, (int)defs.size()); ++g) group_cands.push_back(g);
+    bool sized = want > 0;
     int best = -1, best_g = 0;
     double best_us = 0.0;
     std::vector<float> vp((size_t)c->cfg.ny * c->cfg.nx, 2000.0f);
@@ -1455,6 +1457,14 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
         c->want_groups = 0;
         CHK(setup_shots(c, (int)defs.size(), defs.data()));
         CHK(fwi_set_model(c, vp.data()));
+        if (!sized) {
+            const int a = (int)((c->state.size + kGroupStateBytes - 1) / kGroupStateBytes);
+            for (int g = a - 1; g <= a + 1; ++g)
+                if (g >= 1 && g <= (int)defs.size() &&
+                    std::find(group_cands.begin(), group_cands.end(), g) == group_cands.end())
+                    group_cands.push_back(g);
+            sized = true;
+        }
         for (int G : group_cands) {
             c->want_groups = G;
             CHK(build_groups(c));
