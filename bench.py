@@ -168,10 +168,10 @@ def main():
     tuned = None
     if not a.rows:
         rows, us = ctx.autotune(a.tune_steps)
-        tuned = {"rows_per_task": rows, "shot_groups": ctx.shot_groups(), "step_us": us,
-                 "steps_timed": a.tune_steps}
-        print(f"[bench] autotune: {rows} rows per work unit, {ctx.shot_groups()} shot groups ({us:.1f} us per "
-              f"forward step)", file=sys.stderr, flush=True)
+        tuned = {"rows_per_task": rows, "frame_rows": ctx.frame_rows(), "shot_groups": ctx.shot_groups(),
+                 "step_us": us, "steps_timed": a.tune_steps}
+        print(f"[bench] autotune: {rows} rows per work unit ({ctx.frame_rows()} per frame work unit), "
+              f"{ctx.shot_groups()} shot groups ({us:.1f} us per forward step)", file=sys.stderr, flush=True)
     ctx.set_sources(acq.wavelets)
     ctx.set_model(acq.vp_true)
     ctx.forward(store_snapshots=False)           # observed data from the true model
@@ -229,6 +229,7 @@ def main():
             "nt": acq.nt,
             "snapshots": a.snap,
             "rows_per_task": ctx.rows_per_task(),
+            "frame_rows": ctx.frame_rows(),
             "shot_groups": groups,
             "autotuned": tuned,
         },

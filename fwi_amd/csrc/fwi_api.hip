@@ -117,6 +117,7 @@ struct Cell {
 struct fwi_ctx {
     fwi_config cfg{};
     int dn = 0, NY = 0, nt_store = 0, rows = 32;
+    int frame_rows = 0;                   // rows per frame work unit (0: min(rows, 2))
     float fA[fwi::MAXDN + 1], fB[fwi::MAXDN + 1];
     std::vector<Shot> shots;
     int nrec_total = 0, nsrc_total = 0;
@@ -353,6 +354,8 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
     std::vector<int> rows;
     std::vector<int2> ents;
     const int R = std::min(c->rows, fwi::MAX_TASK_ROWS);
+    // frame tasks are latency-bound: short row chunks give more of them
+    const int RF = c->frame_rows > 0 ? std::min(c->frame_rows, fwi::MAX_TASK_ROWS) : std::min(R, 2);
     for (int i = 0; i < nshots; ++i) {
         Shot& s = c->shots[i];
         const fwi_shot& f = in[i];
@@ -412,7 +415,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         const bool pair = dn <= fwi::XPAIR_MAX_DN && XL + ACC <= 32 && s.NX >= 64;
         const int YT = fwi::TASK_YBAND, PT = fwi::TASK_XPAIR;
         auto xbands = [&](int a, int b) {
-            for (auto& yr : row_range(a, b, R)) {
+            for (auto& yr : row_range(a, b, RF)) {
                 if (pair) {
                     add_task(ftasks[i], PT, XL, XR, yr.first, yr.second);
                 } else {
@@ -423,7 +426,7 @@ int setup_shots(fwi_ctx* c, int nshots, const fwi_shot* in) {
         };
         auto yband = [&](int b0) {
             xbands(b0, b0 + dn);
-            for (auto& yr : row_range(b0, b0 + dn, R))
+            for (auto& yr : row_range(b0, b0 + dn, RF))
                 for (auto& xr : tile_range(XL, XR, fwi::yband_tile_cols(), -1))
                     add_task(ytasks[i], YT, xr.first, xr.second, yr.first, yr.second);
         };
@@ -1438,6 +1441,7 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
     }
     const int k1 = std::min(c->cfg.nt, steps + 1);
     const int row_cands[] = {4, 6, 8, 12, 16, 24, 32};
+    c->frame_rows = 0;                        // frame work units at their default height first
     const int want = c->want_groups;
     // group counts: the setting itself when fixed, else 1..4 and the
     // cache-sized count with its neighbours (known after the first set-up)
@@ -1482,11 +1486,33 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
             if (best < 0 || us < best_us) { best = r; best_g = want > 0 ? want : G; best_us = us; }
         }
     }
+    // then the frame work units' height with the interior's and the groups fixed
+    int best_f = 0;
+    c->rows = best;
+    c->want_groups = best_g;
+    for (int fr : {1, 2, 3, 4, 6, 8}) {
+        if (fr > best && fr != 1 && fr != 2) continue;
+        c->frame_rows = fr;
+        CHK(setup_shots(c, (int)defs.size(), defs.data()));
+        CHK(fwi_set_model(c, vp.data()));
+        HIPCHK(hipMemsetAsync(c->state.base, 0, c->state.size, c->stream));
+        for (int g = 0; g < (int)c->sgroups.size(); ++g)
+            CHK(issue_steps(c, c->d_desc, g, 0, false, 2, std::min(k1, 4), 1, 0, 0));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int g = 0; g < (int)c->sgroups.size(); ++g) CHK(issue_steps(c, c->d_desc, g, 0, false, 2, k1, 1, 0, 0));
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1000.0 * ms / (k1 - 1);
+        if (us < best_us) { best_f = fr; best_us = us; }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->rows = best;
     c->cfg.rows_per_task = best;
     c->want_groups = best_g;
+    c->frame_rows = best_f;
     CHK(setup_shots(c, (int)defs.size(), defs.data()));
     if (best_rows) *best_rows = best;
     if (best_step_us) *best_step_us = best_us;
@@ -1494,6 +1520,11 @@ int fwi_autotune(fwi_ctx* c, int steps, int* best_rows, double* best_step_us) {
 }
 
 int fwi_get_rows_per_task(fwi_ctx* c) { return c ? c->rows : -1; }
+
+int fwi_get_frame_rows(fwi_ctx* c) {
+    if (!c) return -1;
+    return c->frame_rows > 0 ? std::min(c->frame_rows, fwi::MAX_TASK_ROWS) : std::min(std::min(c->rows, fwi::MAX_TASK_ROWS), 2);
+}
 
 int fwi_set_shot_groups(fwi_ctx* c, int groups) {
     if (!c) return fail("null context");

@@ -60,7 +60,7 @@ def lib():
                      "fwi_get_model", "fwi_get_search", "fwi_exact_transpose", "fwi_exact_gradient",
                      "fwi_autotune", "fwi_get_rows_per_task", "fwi_get_sources", "fwi_get_observed",
                      "fwi_time_filter", "fwi_resample_akima", "fwi_set_sources_raw", "fwi_set_observed_raw",
-                     "fwi_source_estimation", "fwi_set_shot_groups", "fwi_get_shot_groups"):
+                     "fwi_source_estimation", "fwi_set_shot_groups", "fwi_get_shot_groups", "fwi_get_frame_rows"):
             getattr(L, name).restype = C.c_int
         L.fwi_apply_step.argtypes = [_P, C.c_double, C.c_int]
         L.beta_calculous_.restype = None
@@ -485,6 +485,10 @@ class Context:
 
     def rows_per_task(self):
         return int(lib().fwi_get_rows_per_task(self.h))
+
+    def frame_rows(self):
+        """Rows per frame (CPML) work unit."""
+        return int(lib().fwi_get_frame_rows(self.h))
 
     def set_shot_groups(self, groups=0):
         """Sweep the shots in `groups` contiguous groups, each through every step (0: sized to the cache)."""

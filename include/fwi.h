@@ -247,13 +247,17 @@ long long fwi_schedule_bytes(fwi_ctx* ctx, int segment_slots);
 /* Autotuner (no counterpart in the reference): times `steps` forward steps of
  * the context's own shots for each candidate number of rows per work unit
  * (and each shot-group count 1..4 and the cache-sized count +-1, unless
- * fwi_set_shot_groups fixed one),
+ * fwi_set_shot_groups fixed one), then the frame work units' height,
  * keeps the fastest and rebuilds the task list.  Call it right after fwi_set_shots: it re-creates the shot state,
  * so the model, sources and data are uploaded afterwards.  Returns the rows
  * chosen and the measured time per step. */
 int fwi_autotune(fwi_ctx* ctx, int steps, int* best_rows, double* best_step_us);
-/* Rows per work unit of the current task list. */
+/* Rows per work unit of the current task list (interior work units), and
+ * rows per frame work unit (default min(rows, 2): the CPML frame's work is
+ * latency-bound, so short work units that run many at once are faster;
+ * fwi_autotune picks it after the interior height). */
 int fwi_get_rows_per_task(fwi_ctx* ctx);
+int fwi_get_frame_rows(fwi_ctx* ctx);
 
 /* Shot groups (no counterpart in the reference, whose MPI ranks each step
  * their own shots to the end, calculate_misfit_gradient.f90:73-360): the
