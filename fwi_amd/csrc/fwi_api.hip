@@ -1149,6 +1149,9 @@ int fwi_create(const fwi_config* cfg, fwi_ctx** out) {
     if (!cfg || !out) return fail("null argument");
     CHK(validate_config(*cfg));
     if (cfg->nt % cfg->store_snap != 0) return fail("nt must be a multiple of store_snap");
+    // the reference's snapshot counter never reaches 1 (see fwi_solver_acoustic),
+    // so store_snap = 1 there means no snapshots and no gradient
+    if (cfg->store_snap < 2) return fail("store_snap must be >= 2 (the reference stores no snapshot at 1)");
     fwi_ctx* c = new fwi_ctx();
     int r = ctx_init(c, cfg, cfg->nt / cfg->store_snap);
     if (r) { delete c; return r; }
@@ -1759,7 +1762,12 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
     cfg.store_snap = store_snap > 0 ? store_snap : 1;
     cfg.dpml = o.dpml; cfg.free_surface = o.free_surface; cfg.snap_kind = FWI_SNAP_F32;
     CHK(validate_config(cfg));
-    const bool snaps = (iprop == 1 || iprop == 2);
+    // The snapshot counter starts at 1 and is incremented before it is compared
+    // with store_snap (solver_acoustic_SS.f90:303,317,440,459), so for
+    // store_snap <= 1 the reference never stores a snapshot (iprop=1 leaves
+    // Fields as it is) and never correlates one (iprop=2 leaves kernel as it is).
+    const bool no_snaps = store_snap <= 1;
+    const bool snaps = (iprop == 1 || iprop == 2) && !no_snaps;
     if (snaps && nt_Store < 0) return fail("negative nt_Store");
     fwi_ctx c;
     CHK(ctx_init(&c, &cfg, snaps ? nt_Store : 0));
@@ -1771,13 +1779,17 @@ int fwi_solver_acoustic(const fwi_solver_opts* opts, int iprop, int NumSou, int 
     sh.nrec = NumRec; sh.rec_x = nxRec; sh.rec_y = nyRec;
     CHK(setup_shots(&c, 1, &sh));
     CHK(fwi_set_model(&c, ctmp));
+    if (iprop == 2 && no_snaps) {              // nothing observable: Rec_point is zeroed above
+        HIPCHK(hipStreamSynchronize(c.stream));
+        return 0;
+    }
     const Shot& s = c.shots[0];
     const size_t slot_elems = (size_t)ny * s.Ws;
     if (iprop != 2) {
         if (NumSou) CHK(upload_traces(&c, S, 0));
-        CHK(forward_sweep(&c, iprop == 1));
+        CHK(forward_sweep(&c, snaps));
         if (NumRec) CHK(download_traces(&c, Rec_point, 1));
-        if (iprop == 1) {
+        if (snaps) {
             HIPCHK(hipStreamSynchronize(c.stream));
             std::vector<float> slot(slot_elems);
             const int last = std::min(nt_Store, nt / cfg.store_snap);

@@ -38,6 +38,10 @@ extern "C" {
  *           kernel(ny,nx) += imaging condition against Fields; Rec_point is
  *           zeroed.
  *   other   forward without snapshots.
+ * As in the reference, snapshots are taken every store_snap steps only for
+ * store_snap >= 2: its step counter starts above 1, so store_snap <= 1 stores
+ * no snapshot (iprop=1 leaves Fields as it is) and correlates none (iprop=2
+ * leaves kernel as it is).
  * CPML thickness 20 cells, free surface on, 6th-order stencil -- the
  * reference's hard-coded values (src/solver_acoustic_SS.f90:90-93).
  * `rank` is accepted and ignored (the reference only uses it for a dead
@@ -111,7 +115,7 @@ typedef struct fwi_config {
     int ny, nx;            /* model rows (depth) and columns */
     float d, dt;           /* grid spacing [m], time step [s] */
     int nt;                /* time samples; must be a multiple of store_snap */
-    int store_snap;        /* snapshot interval in steps (reference: 10) */
+    int store_snap;        /* snapshot interval in steps, >= 2 (reference: 10) */
     int dpml;              /* CPML thickness (reference: 20) */
     int free_surface;      /* as in fwi_solver_opts */
     int snap_kind;         /* FWI_SNAP_F32 / _BF16 / _F16 */

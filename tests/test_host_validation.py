@@ -1,0 +1,163 @@
+"""Argument checks of the C ABI that run before any device work, so they are
+exercised here without a GPU: each bad call returns non-zero and leaves a
+message naming the problem in fwi_last_error().  (The reference routines take
+their arguments unchecked; these are the library's own error returns.)"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fwi_amd", "libfwi_hip.so")
+
+
+class Config(C.Structure):
+    _fields_ = [("ny", C.c_int), ("nx", C.c_int), ("d", C.c_float), ("dt", C.c_float), ("nt", C.c_int),
+                ("store_snap", C.c_int), ("dpml", C.c_int), ("free_surface", C.c_int), ("snap_kind", C.c_int),
+                ("rows_per_task", C.c_int)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("dpml", C.c_int), ("free_surface", C.c_int)]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip("libfwi_hip.so not built")
+    L = C.CDLL(LIB)
+    L.fwi_last_error.restype = C.c_char_p
+    return L
+
+
+def _err(L):
+    return L.fwi_last_error().decode()
+
+
+GOOD = dict(ny=40, nx=60, d=25.0, dt=0.002, nt=200, store_snap=10, dpml=20, free_surface=1, snap_kind=0,
+            rows_per_task=0)
+
+
+@pytest.mark.parametrize("change, message", [
+    (dict(ny=0), "model dimensions must be positive"),
+    (dict(nx=-3), "model dimensions must be positive"),
+    (dict(nt=1), "nt must be at least 2"),
+    (dict(store_snap=0), "store_snap must be positive"),
+    (dict(store_snap=1), "store_snap must be >= 2"),
+    (dict(dpml=1), "dpml must be >= 2"),
+    (dict(dpml=53), "dpml out of range"),           # dpml + 6 > 58 cells of frame
+    (dict(d=0.0), "d and dt must be positive"),
+    (dict(dt=-0.001), "d and dt must be positive"),
+    (dict(d=math.nan), "d and dt must be positive"),
+    (dict(snap_kind=3), "unknown snapshot kind"),
+    (dict(rows_per_task=-1), "rows_per_task must be >= 0"),
+    (dict(nt=205), "nt must be a multiple of store_snap"),
+])
+def test_create_rejects_bad_config(lib, change, message):
+    cfg = Config(**{**GOOD, **change})
+    h = C.c_void_p()
+    assert lib.fwi_create(C.byref(cfg), C.byref(h)) != 0
+    assert message in _err(lib)
+    assert not h.value
+
+
+def test_create_null_arguments(lib):
+    h = C.c_void_p()
+    assert lib.fwi_create(None, C.byref(h)) != 0 and "null argument" in _err(lib)
+    cfg = Config(**GOOD)
+    assert lib.fwi_create(C.byref(cfg), None) != 0 and "null argument" in _err(lib)
+
+
+def test_context_calls_on_null_handle(lib):
+    buf = np.zeros(16, np.float32)
+    p = buf.ctypes.data_as(C.c_void_p)
+    d = C.c_double()
+    checks = [
+        (lambda: lib.fwi_set_shots(None, 1, p), "bad arguments to fwi_set_shots"),
+        (lambda: lib.fwi_set_model(None, p), "set shots first"),
+        (lambda: lib.fwi_set_sources(None, p), "set shots first"),
+        (lambda: lib.fwi_forward(None, 0), "model not set"),
+        (lambda: lib.fwi_get_synthetic(None, p), "no synthetic traces yet"),
+        (lambda: lib.fwi_misfit(None, C.byref(d)), "run fwi_forward first"),
+        (lambda: lib.fwi_adjoint(None), "run fwi_forward with snapshots first"),
+        (lambda: lib.fwi_reduce_gradient(None, C.c_double(1.0), None), "no shots"),
+        (lambda: lib.fwi_set_shot_groups(None, 1), "null context"),
+        (lambda: lib.fwi_set_schedule(None, 0), "null context"),
+        (lambda: lib.fwi_plan_schedule(None, C.c_longlong(1 << 30), None), "set shots first"),
+        (lambda: lib.fwi_autotune(None, 10, None, None), "set shots first"),
+        (lambda: lib.fwi_nlcg_direction(None, 1, 1, C.byref(d)), "no shots"),
+        (lambda: lib.fwi_apply_step(None, C.c_double(0.1), 0), "no model"),
+    ]
+    for call, message in checks:
+        assert call() != 0
+        assert message in _err(lib), (message, _err(lib))
+    assert lib.fwi_get_shot_groups(None) == -1
+    lib.fwi_device_bytes.restype = C.c_longlong
+    assert lib.fwi_device_bytes(None) == 0
+
+
+def _solver(L, iprop=1, nsou=1, nrec=1, fields=True, kernel=True, dpml=20, nt_store=20):
+    ny, nx, nt, ss = 8, 8, 20, 10
+    ix = np.ones(max(1, nsou, nrec), np.int32)
+    S = np.zeros(nt * max(1, nsou), np.float32)
+    c = np.full(ny * nx, 1500.0, np.float32)
+    F = np.zeros(ny * nx * (nt // ss + 1), np.float32) if fields else None
+    K = np.zeros(ny * nx, np.float32) if kernel else None
+    R = np.zeros(nt * max(1, nrec), np.float32)
+    ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    opts = Opts(dpml, 1)
+    return L.fwi_solver_acoustic(C.byref(opts), iprop, nsou, nrec, ptr(ix), ptr(ix), ptr(ix), ptr(ix), ptr(S), ptr(S),
+                                 C.c_float(25.0), C.c_float(0.002), ny, nx, ptr(c), ss, nt_store, nt, ptr(K), ptr(F),
+                                 ptr(R))
+
+
+@pytest.mark.parametrize("kw, message", [
+    (dict(iprop=1, fields=False), "iprop=1 needs Fields"),
+    (dict(iprop=2, kernel=False), "iprop=2 needs Fields and kernel"),
+    (dict(iprop=2, fields=False), "iprop=2 needs Fields and kernel"),
+    (dict(nsou=-1), "negative source/receiver count"),
+    (dict(nrec=-2), "negative source/receiver count"),
+    (dict(dpml=1), "dpml must be >= 2"),
+    (dict(dpml=60), "dpml out of range"),
+    (dict(nt_store=-1), "negative nt_Store"),
+])
+def test_dropin_solver_rejects_bad_arguments(lib, kw, message):
+    assert _solver(lib, **kw) != 0
+    assert message in _err(lib)
+
+
+def test_trace_routines_argument_checks(lib):
+    x = np.ones(64, np.float32)
+    p = x.ctypes.data_as(C.c_void_p)
+    f = C.c_float
+    assert lib.fwi_time_filter(p, 0, f(0.002), 1, 3, f(3.0), f(5.0)) != 0 and "bad arguments" in _err(lib)
+    assert lib.fwi_time_filter(p, 64, f(0.0), 1, 3, f(3.0), f(5.0)) != 0 and "bad arguments" in _err(lib)
+    for typef in (0, 4):
+        assert lib.fwi_time_filter(p, 64, f(0.002), 1, typef, f(3.0), f(5.0)) != 0
+        assert "typef must be 1, 2 or 3" in _err(lib)
+    # no traces: nothing to do, no device work, data untouched
+    assert lib.fwi_time_filter(p, 64, f(0.002), 0, 3, f(3.0), f(5.0)) == 0 and (x == 1).all()
+    assert lib.fwi_resample_akima(p, f(0.004), 1, p, f(0.002), 10, 1) != 0 and "bad arguments" in _err(lib)
+    assert lib.fwi_resample_akima(p, f(0.004), 8, p, f(0.002), 0, 1) != 0 and "bad arguments" in _err(lib)
+    assert lib.fwi_resample_akima(p, f(0.004), 8, p, f(0.002), 10, 0) == 0 and (x == 1).all()
+
+
+def test_source_estimation_argument_checks(lib):
+    nx, ny, nt, nr = 30, 20, 100, 3
+    obs = np.zeros(nt * nr, np.float32)
+    model = np.full(nx * ny, 1500.0, np.float32)
+    nxr = np.array([5, 10, 15], np.int32)
+    nyr = np.ones(nr, np.int32)
+    sst = np.zeros(nt, np.float32)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    f = C.c_float
+
+    def call(nx_=nx, nt_=nt, nr_=nr, nxs=10):
+        return lib.fwi_source_estimation(ptr(obs), f(3.0), f(8.0), nx_, ny, nt_, nr_, f(25.0), f(0.002), nxs, 1,
+                                         ptr(nxr), ptr(nyr), ptr(model), ptr(sst))
+    for kw in (dict(nt_=1), dict(nr_=0), dict(nx_=0)):
+        assert call(**kw) != 0 and "bad sizes" in _err(lib)
+    for nxs in (0, nx + 1):
+        assert call(nxs=nxs) != 0 and "source column outside the model" in _err(lib)
