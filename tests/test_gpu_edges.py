@@ -193,3 +193,21 @@ def test_dropin_store_snap_below_two(ss):
     if rb.ref_available("fs0_r4"):
         R = rb.ref_solver("fs0_r4", 1, nxs, nys, nxr, nyr, S, None, D, DT, c, ss, nts, nt)
         assert np.array_equal(R.rec, rec) and np.abs(R.fields).max() == 0
+
+
+@pytest.mark.parametrize("nt, ss, nts", [(205, 10, 20), (200, 10, 25), (123, 7, 17)])
+def test_dropin_record_not_a_multiple_of_store_snap(nt, ss, nts):
+    """The reference's caller rounds nt down to a multiple of store_snap, but the
+    routine itself takes any nt and nt_Store: snapshots at every store_snap-th
+    step into slots 1, 2, ...; the adjoint correlates slot nt_Store - k/store_snap
+    at step k (slots past the stored ones hold what the caller passed)."""
+    c, S, nxs, nys, nxr, nyr = synth.small_case(6, 36, 52, nt)
+    rec, F, _ = fwi_amd.solver_acoustic_SS(1, nxs, nys, nxr, nyr, S, None, D, DT, c, ss, nts, nt, free_surface=0)
+    O = rb.oracle_solver("f32", 1, nxs, nys, nxr, nyr, S, None, D, DT, c, ss, nts, nt, free_surface=0)
+    assert np.array_equal(rec, O.rec) and np.array_equal(F, O.fields)
+    adj = -O.rec[::-1].copy()
+    _, _, k = fwi_amd.solver_acoustic_SS(2, nxs, nys, nxr, nyr, S, adj, D, DT, c, ss, nts, nt,
+                                         Fields=O.fields.copy(order="F"), free_surface=0)
+    O2 = rb.oracle_solver("f32", 2, nxs, nys, nxr, nyr, S, adj, D, DT, c, ss, nts, nt, fields=O.fields,
+                          free_surface=0)
+    assert np.abs(O2.kernel).max() > 0 and np.array_equal(k, O2.kernel)
