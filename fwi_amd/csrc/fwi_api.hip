@@ -659,10 +659,10 @@ int num_segments(const fwi_ctx* c, int seg) {
 int slots_for(const fwi_ctx* c, int seg) { return seg > 0 ? std::min(seg, std::max(c->nt_store - 1, 0)) : c->nt_store; }
 
 // Device bytes of the snapshot storage of a schedule: slots (+ zero slot),
-// FP32 staging for the 16-bit kinds, checkpoints and the adjoint state.
+// FP32 staging and scales for FP16, checkpoints and the adjoint state.
 size_t storage_bytes(const fwi_ctx* c, int seg) {
     const size_t ny = c->cfg.ny;
-    const bool q16 = c->cfg.snap_kind != fwi::SNAP_F32;
+    const bool q16 = c->cfg.snap_kind == fwi::SNAP_F16;
     const size_t nsl = slots_for(c, seg) + 1;
     size_t b = 0;
     for (const Shot& s : c->shots) {
@@ -680,7 +680,7 @@ int attach_storage(fwi_ctx* c) {
     if (c->storage_ready) return 0;
     const int ns = (int)c->shots.size();
     const size_t ny = c->cfg.ny;
-    const bool q16 = c->cfg.snap_kind != fwi::SNAP_F32;
+    const bool q16 = c->cfg.snap_kind == fwi::SNAP_F16;     // staged and scaled (BF16 is written directly)
     c->nslots = slots_for(c, c->seg);
     const size_t nsl = c->nslots + 1;
     std::vector<size_t> o_stage(ns), o_scale(ns);
@@ -796,7 +796,7 @@ int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int grp, int mode, bool re
     std::memcpy(prm.fA, c->fA, sizeof(prm.fA));
     std::memcpy(prm.fB, c->fB, sizeof(prm.fB));
     const int ss = cf.store_snap;
-    const bool q16 = cf.snap_kind != fwi::SNAP_F32;
+    const bool q16 = cf.snap_kind == fwi::SNAP_F16;
     for (int k = std::max(k0, 2); k <= k1; ++k) {
         prm.k = k;
         prm.snap_op = 0;
@@ -822,8 +822,7 @@ int issue_steps(fwi_ctx* c, const fwi::ShotDev* desc, int grp, int mode, bool re
         if (hist)
             HIPCHK(fwi::launch_hist_store(desc, c->d_exact, (int)c->shots.size(), c->exact_max_stride, k, c->stream));
         if (prm.snap_op == 1 && q16)
-            HIPCHK(fwi::launch_quantize(c->d_quant + q0, nq, c->max_slot_elems, prm.slot, cf.snap_kind == fwi::SNAP_BF16,
-                                        c->stream));
+            HIPCHK(fwi::launch_quantize(c->d_quant + q0, nq, c->max_slot_elems, prm.slot, c->stream));
     }
     return 0;
 }

@@ -27,18 +27,18 @@ struct GradShot {     // one shot's FP32 imaging kernel, model column m at k[row
     int x0, W, Ws, moff;
 };
 
-struct QuantShot {    // one shot's staged FP32 snapshot and its 16-bit slots
+struct QuantShot {    // one shot's staged FP32 snapshot and its FP16 slots
     const float* stage;           // [ny][Ws]
     unsigned short* slots;        // slot s at slots + s * stride
     float* scale;                 // per-slot scale factor
     const unsigned int* maxbits;  // per-slot max |value| (float bits)
-    size_t stride;                // elements per slot
+    size_t stride;                // elements per slot, a multiple of 4
 };
 
-// Quantises every shot's staging slot into 16-bit slot `slot` (bf16 != 0:
-// bfloat16, else IEEE half) with a power-of-two scale taken from the slot's
-// maximum, and records the scale.
-hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, int bf16, hipStream_t st);
+// Quantises every shot's staging slot into FP16 slot `slot` with a
+// power-of-two scale taken from the slot's maximum, and records the scale.
+// (BF16 snapshots need no scale and are written by the step kernel itself.)
+hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, hipStream_t st);
 // Fills n 32-bit words with v (a kernel, so it may sit in a captured graph).
 hipError_t launch_fill_u32(unsigned int* p, size_t n, unsigned int v, hipStream_t st);
 // Copies n16 16-byte words.

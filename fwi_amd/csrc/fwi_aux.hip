@@ -140,14 +140,21 @@ __device__ __forceinline__ float snap_scale_of(unsigned int bits) {
     return ldexpf(1.0f, e - 15);
 }
 
-__global__ void quantize_kernel(const QuantShot* __restrict__ shots, int slot, int bf16) {
+// Four values per thread: the stride is a multiple of 4 elements, the
+// staging slot 16-byte and every 16-bit slot 8-byte aligned.
+__device__ __forceinline__ unsigned int half2_bits(float a, float b) {
+    return (unsigned int)__half_as_ushort(__float2half(a)) | ((unsigned int)__half_as_ushort(__float2half(b)) << 16);
+}
+__global__ void quantize_kernel(const QuantShot* __restrict__ shots, int slot) {
     const QuantShot q = shots[blockIdx.y];
     const float sc = snap_scale_of(q.maxbits[slot]);
     const float inv = 1.0f / sc;
-    unsigned short* __restrict__ out = q.slots + (size_t)slot * q.stride;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < q.stride; e += (size_t)gridDim.x * blockDim.x) {
-        const float v = q.stage[e] * inv;
-        out[e] = bf16 ? __bfloat16_as_ushort(__float2bfloat16(v)) : __half_as_ushort(__float2half(v));
+    const float4* __restrict__ in = reinterpret_cast<const float4*>(q.stage);
+    uint2* __restrict__ out = reinterpret_cast<uint2*>(q.slots + (size_t)slot * q.stride);
+    const size_t n4 = q.stride / 4;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n4; e += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = in[e];
+        out[e] = make_uint2(half2_bits(v.x * inv, v.y * inv), half2_bits(v.z * inv, v.w * inv));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) q.scale[slot] = sc;
 }
@@ -303,10 +310,10 @@ hipError_t launch_base_model(const float* vp, float* lam0, size_t n, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, int bf16, hipStream_t st) {
+hipError_t launch_quantize(const QuantShot* shots, int nshots, size_t max_stride, int slot, hipStream_t st) {
     if (nshots <= 0) return hipSuccess;
-    const unsigned bx = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (max_stride + 1023) / 1024));
-    hipLaunchKernelGGL(quantize_kernel, dim3(bx, nshots), dim3(256), 0, st, shots, slot, bf16);
+    const unsigned bx = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (max_stride / 4 + 255) / 256));
+    hipLaunchKernelGGL(quantize_kernel, dim3(bx, nshots), dim3(256), 0, st, shots, slot);
     return hipGetLastError();
 }
 

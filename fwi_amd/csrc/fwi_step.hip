@@ -95,6 +95,7 @@ __device__ __forceinline__ float from_right(float v) {
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef int i2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float4 ld4(const GMEM float* p) {
     const f4v v = *reinterpret_cast<const GMEM f4v*>(p);
     return make_float4(v.x, v.y, v.z, v.w);
@@ -104,11 +105,19 @@ __device__ __forceinline__ void st4(GMEM float* p, float4 v) {
     *reinterpret_cast<GMEM f4v*>(p) = w;
 }
 
+// 16-bit snapshot values.  BF16 slots hold the value rounded to bfloat16
+// (nearest even) and carry no scale: bfloat16 has FP32's exponent range, so a
+// power-of-two scale would change no bit of a normal value.  FP16 slots hold
+// value / scale with the slot's power-of-two scale.
+__device__ __forceinline__ float snap_from_bits(unsigned short b, int kind, float scale) {
+    if (kind == SNAP_BF16) return __bfloat162float(__ushort_as_bfloat16(b));
+    return __half2float(__ushort_as_half(b)) * scale;
+}
+__device__ __forceinline__ unsigned short to_bf16_bits(float v) { return __bfloat16_as_ushort(__float2bfloat16(v)); }
+
 __device__ __forceinline__ float snap_load(const GMEM void* base, size_t idx, int kind, float scale) {
     if (kind == SNAP_F32) return static_cast<const GMEM float*>(base)[idx];
-    const unsigned short b = static_cast<const GMEM unsigned short*>(base)[idx];
-    if (kind == SNAP_BF16) return __bfloat162float(__ushort_as_bfloat16(b)) * scale;
-    return __half2float(__ushort_as_half(b)) * scale;
+    return snap_from_bits(static_cast<const GMEM unsigned short*>(base)[idx], kind, scale);
 }
 
 __device__ __forceinline__ GMEM void* slot_base(const GMEM ShotDev& s, const StepParams& prm) {
@@ -116,11 +125,24 @@ __device__ __forceinline__ GMEM void* slot_base(const GMEM ShotDev& s, const Ste
 }
 
 // Where a storing step writes its FP32 snapshot values: the slot itself for
-// the FP32 kind, the shot's staging slot for the 16-bit kinds (quantised
-// after the step with a scale taken from the slot's largest |value|).
+// the FP32 kind, the shot's staging slot for FP16 (quantised after the step
+// with a scale taken from the slot's largest |value|).  BF16 values go to the
+// slot directly (snap_store).
 __device__ __forceinline__ GMEM float* store_base(const GMEM ShotDev& s, const StepParams& prm) {
     if (prm.snap_kind == SNAP_F32) return static_cast<GMEM float*>(slot_base(s, prm));
-    return gmem(s.stage);
+    if (prm.snap_kind == SNAP_F16) return gmem(s.stage);
+    return nullptr;
+}
+
+// One snapshot value of a storing step at model cell mi.
+__device__ __forceinline__ void snap_store(GMEM float* store, GMEM void* slot, size_t mi, int kind, float f,
+                                           float& vmax) {
+    if (kind == SNAP_BF16) {
+        static_cast<GMEM unsigned short*>(slot)[mi] = to_bf16_bits(f);
+        return;
+    }
+    store[mi] = f;
+    vmax = fmaxf(vmax, fabsf(f));
 }
 
 // Folds the wavefront's largest stored |value| into the slot maximum (float
@@ -208,11 +230,9 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
     float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
     GMEM float* store = nullptr;
+    if (prm.snap_op != 0) snap_slot = slot_base(s, prm);
     if (prm.snap_op == 1) store = store_base(s, prm);
-    if (prm.snap_op == 2) {
-        snap_slot = slot_base(s, prm);
-        if (prm.snap_kind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
-    }
+    if (prm.snap_op == 2 && prm.snap_kind == SNAP_F16) snap_scale = gmem(s.snap_scale)[prm.slot];
 
     L.src[lane] = 0.0f;
 
@@ -347,9 +367,7 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
             if (prm.snap_op != 0 && model_row && model_col) {
                 const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
                 if (prm.snap_op == 1) {
-                    const float f = ((pp - pmr) * 0.5f) / prm.dt;
-                    store[mi] = f;
-                    vmax = fmaxf(vmax, fabsf(f));
+                    snap_store(store, snap_slot, mi, prm.snap_kind, ((pp - pmr) * 0.5f) / prm.dt, vmax);
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
@@ -393,7 +411,7 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
         }
         buf ^= 1;
     }
-    if (prm.snap_op == 1 && prm.snap_kind != SNAP_F32) snap_max_commit(s, prm, vmax);
+    if (prm.snap_op == 1 && prm.snap_kind == SNAP_F16) snap_max_commit(s, prm, vmax);
 }
 
 // Value of v held by lane `src` (all 64 lanes active).
@@ -457,11 +475,9 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
     float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
     GMEM float* store = nullptr;
+    if (prm.snap_op != 0) snap_slot = slot_base(s, prm);
     if (prm.snap_op == 1) store = store_base(s, prm);
-    if (prm.snap_op == 2) {
-        snap_slot = slot_base(s, prm);
-        if (prm.snap_kind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
-    }
+    if (prm.snap_op == 2 && prm.snap_kind == SNAP_F16) snap_scale = gmem(s.snap_scale)[prm.slot];
 
     L.src[lane] = 0.0f;
 
@@ -592,9 +608,7 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
             if (prm.snap_op != 0 && model_row && model_col) {
                 const size_t mi = (size_t)(y - dn) * s.Ws + (c - dn + s.moff);
                 if (prm.snap_op == 1) {
-                    const float f = ((pp - pmr) * 0.5f) / prm.dt;
-                    store[mi] = f;
-                    vmax = fmaxf(vmax, fabsf(f));
+                    snap_store(store, snap_slot, mi, prm.snap_kind, ((pp - pmr) * 0.5f) / prm.dt, vmax);
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
@@ -640,7 +654,7 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
             rc = rc1;
         }
     }
-    if (prm.snap_op == 1 && prm.snap_kind != SNAP_F32) snap_max_commit(s, prm, vmax);
+    if (prm.snap_op == 1 && prm.snap_kind == SNAP_F16) snap_max_commit(s, prm, vmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -678,11 +692,9 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
     float snap_scale = 1.0f, vmax = 0.0f;
     GMEM void* snap_slot = nullptr;
     GMEM float* store = nullptr;
+    if (snap_op != 0) snap_slot = slot_base(s, prm);
     if (snap_op == 1) store = store_base(s, prm);
-    if (snap_op == 2) {
-        snap_slot = slot_base(s, prm);
-        if (skind != SNAP_F32) snap_scale = gmem(s.snap_scale)[prm.slot];
-    }
+    if (snap_op == 2 && skind == SNAP_F16) snap_scale = gmem(s.snap_scale)[prm.slot];
     const int mcol = xl - dn + s.moff;      // model storage column of this lane's first cell
     const GMEM float* lanes_p = pc + xl;
     GMEM float* lanes_pn = pn + xl;
@@ -781,21 +793,39 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
                     f.y = ((pp.y - pm.y) * 0.5f) / prm.dt;
                     f.z = ((pp.z - pm.z) * 0.5f) / prm.dt;
                     f.w = ((pp.w - pm.w) * 0.5f) / prm.dt;
-                    st4(store + mi, f);
-                    vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+                    if (skind == SNAP_BF16) {
+                        // 4 values in one 8-byte store (mi is a multiple of 4)
+                        const unsigned int lo = to_bf16_bits(f.x) | ((unsigned int)to_bf16_bits(f.y) << 16);
+                        const unsigned int hi = to_bf16_bits(f.z) | ((unsigned int)to_bf16_bits(f.w) << 16);
+                        const u2v w = {lo, hi};
+                        *reinterpret_cast<GMEM u2v*>(static_cast<GMEM unsigned short*>(snap_slot) + mi) = w;
+                    } else {
+                        st4(store + mi, f);
+                        vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
+                    }
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (xl + i < t.xe) {
-                            const float f = ((ppv[i] - pmv[i]) * 0.5f) / prm.dt;
-                            store[mi + i] = f;
-                            vmax = fmaxf(vmax, fabsf(f));
-                        }
+                        if (xl + i < t.xe) snap_store(store, snap_slot, mi + i, skind, ((ppv[i] - pmv[i]) * 0.5f) / prm.dt, vmax);
                 }
             } else {
                 GMEM float* g = grad + mi;
                 if (full && skind == SNAP_F32) {
                     const float4 f = ld4(static_cast<const GMEM float*>(snap_slot) + mi);
+                    float4 gv = ld4(g);
+                    gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
+                    gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
+                    gv.z = gv.z + (-(((0.5f * (pp.z - pm.z)) * f.z) * prm.ssf));
+                    gv.w = gv.w + (-(((0.5f * (pp.w - pm.w)) * f.w) * prm.ssf));
+                    st4(g, gv);
+                } else if (full) {
+                    // 16-bit kinds: 4 values in one 8-byte load (mi is a multiple of 4)
+                    const u2v u = *reinterpret_cast<const GMEM u2v*>(static_cast<const GMEM unsigned short*>(snap_slot) + mi);
+                    float4 f;
+                    f.x = snap_from_bits((unsigned short)(u.x & 0xffffu), skind, snap_scale);
+                    f.y = snap_from_bits((unsigned short)(u.x >> 16), skind, snap_scale);
+                    f.z = snap_from_bits((unsigned short)(u.y & 0xffffu), skind, snap_scale);
+                    f.w = snap_from_bits((unsigned short)(u.y >> 16), skind, snap_scale);
                     float4 gv = ld4(g);
                     gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
                     gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
@@ -836,7 +866,7 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
         if (!row(y0, std::integral_constant<int, 6>{})) break;
         if (!row(y0, std::integral_constant<int, 7>{})) break;
     }
-    if (snap_op == 1 && skind != SNAP_F32) snap_max_commit(s, prm, vmax);
+    if (snap_op == 1 && skind == SNAP_F16) snap_max_commit(s, prm, vmax);
 }
 
 // ---------------------------------------------------------------------------
