@@ -14,6 +14,10 @@ Prints one JSON line.  `value` is credited cell-updates per second
 of all shots divided by the time of that step (HIP events on the context's
 stream around the sweeps; a step of all shots is one launch per shot group,
 groups being swept one after the other).
+`roofline.traffic` is the kernel's HBM bytes per launch measured with
+rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, each scaled by its reading on
+a known-size copy; profiles/step_kernel_marine64_2000x400_counters.json),
+reported when this run's decomposition is the profiled one, else null.
 `cpu_baseline` times the reference's own Fortran solver (compiled from its
 sources into oracle/_ref) on a bounded sample of the same workload, one
 process per core as `mpirun -np C` over shots would run it.
@@ -31,6 +35,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s fwd+adj (% HBM roofline); seconds per 64-shot FWI gradient"
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(cfg):
+    """HBM bytes per step_kernel launch from the committed counter runs, when the
+    run's decomposition (shots, snapshot kind, work-unit heights, shot groups)
+    is the profiled one; (None, None) otherwise."""
+    path = os.path.join(ROOT, "profiles", "step_kernel_marine64_2000x400_counters.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if any(cfg.get(k) != v for k, v in rec["config"].items()):
+        return None, None
+    kib = (rec["fetch_size_kib_per_dispatch"] * rec["fetch_factor"]
+           + rec["write_size_kib_per_dispatch"] * rec["write_factor"])
+    return kib * 1024.0, os.path.relpath(path, ROOT)
 
 
 def parse():
@@ -208,6 +229,9 @@ def main():
     avg_step_s = step_ms / 1e3 / max(nlaunch, 1)
     bytes_per_step = bytes_per_grad / steps_per_grad
     achieved = bytes_per_step / avg_step_s / 1e9
+    traffic, traffic_src = measured_traffic({"nshots": len(acq.shots), "snapshots": a.snap,
+                                             "rows_per_task": ctx.rows_per_task(), "frame_rows": ctx.frame_rows(),
+                                             "shot_groups": groups})
     out = {
         "metric": METRIC,
         "value": gcell,
@@ -242,7 +266,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "traffic_over_algorithmic": (traffic / (bytes_per_step / groups)) if traffic else None,
             "algorithmic_bytes_per_step": bytes_per_step,
             "avg_step_us": avg_step_s * 1e6,
             "launches_per_step": groups,
