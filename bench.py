@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 def measured_traffic(cfg):
-    """HBM bytes per step_kernel launch from the committed counter runs, when the
-    run's decomposition (shots, snapshot kind, work-unit heights, shot groups)
-    is the profiled one; (None, None) otherwise."""
+    """HBM bytes per step_kernel launch from the committed counter runs and the
+    profile record, when the run's decomposition (shots, snapshot kind,
+    work-unit heights, shot groups) is the profiled one; (None, None) otherwise."""
     path = os.path.join(ROOT, "profiles", "step_kernel_marine64_2000x400_counters.json")
     try:
         with open(path) as f:
@@ -51,7 +51,7 @@ def measured_traffic(cfg):
         return None, None
     kib = (rec["fetch_size_kib_per_dispatch"] * rec["fetch_factor"]
            + rec["write_size_kib_per_dispatch"] * rec["write_factor"])
-    return kib * 1024.0, os.path.relpath(path, ROOT)
+    return kib * 1024.0, rec
 
 
 def parse():
@@ -229,7 +229,7 @@ def main():
     avg_step_s = step_ms / 1e3 / max(nlaunch, 1)
     bytes_per_step = bytes_per_grad / steps_per_grad
     achieved = bytes_per_step / avg_step_s / 1e9
-    traffic, traffic_src = measured_traffic({"nshots": len(acq.shots), "snapshots": a.snap,
+    traffic, prof = measured_traffic({"nshots": len(acq.shots), "snapshots": a.snap,
                                              "rows_per_task": ctx.rows_per_task(), "frame_rows": ctx.frame_rows(),
                                              "shot_groups": groups})
     out = {
@@ -267,8 +267,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": traffic_src,
+            "traffic_source": "profiles/step_kernel_marine64_2000x400_counters.json" if prof else None,
             "traffic_over_algorithmic": (traffic / (bytes_per_step / groups)) if traffic else None,
+            # the events span whole sweeps, so avg_launch_us includes the gaps between
+            # graph nodes; the profiled kernel-only average of the same decomposition:
+            "rocprof_kernel_avg_us": prof["kernel_trace_avg_us"] if prof else None,
             "algorithmic_bytes_per_step": bytes_per_step,
             "avg_step_us": avg_step_s * 1e6,
             "launches_per_step": groups,
