@@ -150,11 +150,14 @@ __global__ void quantize_kernel(const QuantShot* __restrict__ shots, int slot) {
     const float sc = snap_scale_of(q.maxbits[slot]);
     const float inv = 1.0f / sc;
     const float4* __restrict__ in = reinterpret_cast<const float4*>(q.stage);
-    uint2* __restrict__ out = reinterpret_cast<uint2*>(q.slots + (size_t)slot * q.stride);
+    typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+    u2v* __restrict__ out = reinterpret_cast<u2v*>(q.slots + (size_t)slot * q.stride);
     const size_t n4 = q.stride / 4;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n4; e += (size_t)gridDim.x * blockDim.x) {
         const float4 v = in[e];
-        out[e] = make_uint2(half2_bits(v.x * inv, v.y * inv), half2_bits(v.z * inv, v.w * inv));
+        // the slot is read again only by the adjoint: non-temporal store
+        const u2v w = {half2_bits(v.x * inv, v.y * inv), half2_bits(v.z * inv, v.w * inv)};
+        __builtin_nontemporal_store(w, out + e);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) q.scale[slot] = sc;
 }

@@ -105,6 +105,21 @@ __device__ __forceinline__ void st4(GMEM float* p, float4 v) {
     *reinterpret_cast<GMEM f4v*>(p) = w;
 }
 
+// Snapshot slots and imaging kernels are touched once per snapshot interval;
+// their loads and stores carry the non-temporal hint, so that this traffic
+// does not push the shot group's propagation state out of the caches (the
+// step after a storing or imaging step then runs at the plain steps' speed).
+template <class T> __device__ __forceinline__ T ld_nt(const GMEM T* p) { return __builtin_nontemporal_load(p); }
+template <class T> __device__ __forceinline__ void st_nt(GMEM T* p, T v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float4 ld4_nt(const GMEM float* p) {
+    const f4v v = ld_nt(reinterpret_cast<const GMEM f4v*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4_nt(GMEM float* p, float4 v) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    st_nt(reinterpret_cast<GMEM f4v*>(p), w);
+}
+
 // 16-bit snapshot values.  BF16 slots hold the value rounded to bfloat16
 // (nearest even) and carry no scale: bfloat16 has FP32's exponent range, so a
 // power-of-two scale would change no bit of a normal value.  FP16 slots hold
@@ -116,8 +131,8 @@ __device__ __forceinline__ float snap_from_bits(unsigned short b, int kind, floa
 __device__ __forceinline__ unsigned short to_bf16_bits(float v) { return __bfloat16_as_ushort(__float2bfloat16(v)); }
 
 __device__ __forceinline__ float snap_load(const GMEM void* base, size_t idx, int kind, float scale) {
-    if (kind == SNAP_F32) return static_cast<const GMEM float*>(base)[idx];
-    return snap_from_bits(static_cast<const GMEM unsigned short*>(base)[idx], kind, scale);
+    if (kind == SNAP_F32) return ld_nt(static_cast<const GMEM float*>(base) + idx);
+    return snap_from_bits(ld_nt(static_cast<const GMEM unsigned short*>(base) + idx), kind, scale);
 }
 
 __device__ __forceinline__ GMEM void* slot_base(const GMEM ShotDev& s, const StepParams& prm) {
@@ -138,10 +153,10 @@ __device__ __forceinline__ GMEM float* store_base(const GMEM ShotDev& s, const S
 __device__ __forceinline__ void snap_store(GMEM float* store, GMEM void* slot, size_t mi, int kind, float f,
                                            float& vmax) {
     if (kind == SNAP_BF16) {
-        static_cast<GMEM unsigned short*>(slot)[mi] = to_bf16_bits(f);
+        st_nt(static_cast<GMEM unsigned short*>(slot) + mi, to_bf16_bits(f));
         return;
     }
-    store[mi] = f;
+    st_nt(store + mi, f);
     vmax = fmaxf(vmax, fabsf(f));
 }
 
@@ -371,7 +386,7 @@ __device__ __forceinline__ void frame_task(const StepParams& prm, const Task& t,
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
-                    grad[mi] = grad[mi] + k2;
+                    st_nt(grad + mi, ld_nt(grad + mi) + k2);
                 }
             }
         }
@@ -612,7 +627,7 @@ __device__ __forceinline__ void xpair_task(const StepParams& prm, const Task& t,
                 } else {
                     const float f = snap_load(snap_slot, mi, prm.snap_kind, snap_scale);
                     const float k2 = -(((0.5f * (pp - pmr)) * f) * prm.ssf);
-                    grad[mi] = grad[mi] + k2;
+                    st_nt(grad + mi, ld_nt(grad + mi) + k2);
                 }
             }
         }
@@ -798,9 +813,9 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
                         const unsigned int lo = to_bf16_bits(f.x) | ((unsigned int)to_bf16_bits(f.y) << 16);
                         const unsigned int hi = to_bf16_bits(f.z) | ((unsigned int)to_bf16_bits(f.w) << 16);
                         const u2v w = {lo, hi};
-                        *reinterpret_cast<GMEM u2v*>(static_cast<GMEM unsigned short*>(snap_slot) + mi) = w;
+                        st_nt(reinterpret_cast<GMEM u2v*>(static_cast<GMEM unsigned short*>(snap_slot) + mi), w);
                     } else {
-                        st4(store + mi, f);
+                        st4_nt(store + mi, f);
                         vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
                     }
                 } else {
@@ -811,33 +826,33 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
             } else {
                 GMEM float* g = grad + mi;
                 if (full && skind == SNAP_F32) {
-                    const float4 f = ld4(static_cast<const GMEM float*>(snap_slot) + mi);
-                    float4 gv = ld4(g);
+                    const float4 f = ld4_nt(static_cast<const GMEM float*>(snap_slot) + mi);
+                    float4 gv = ld4_nt(g);
                     gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
                     gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
                     gv.z = gv.z + (-(((0.5f * (pp.z - pm.z)) * f.z) * prm.ssf));
                     gv.w = gv.w + (-(((0.5f * (pp.w - pm.w)) * f.w) * prm.ssf));
-                    st4(g, gv);
+                    st4_nt(g, gv);
                 } else if (full) {
                     // 16-bit kinds: 4 values in one 8-byte load (mi is a multiple of 4)
-                    const u2v u = *reinterpret_cast<const GMEM u2v*>(static_cast<const GMEM unsigned short*>(snap_slot) + mi);
+                    const u2v u = ld_nt(reinterpret_cast<const GMEM u2v*>(static_cast<const GMEM unsigned short*>(snap_slot) + mi));
                     float4 f;
                     f.x = snap_from_bits((unsigned short)(u.x & 0xffffu), skind, snap_scale);
                     f.y = snap_from_bits((unsigned short)(u.x >> 16), skind, snap_scale);
                     f.z = snap_from_bits((unsigned short)(u.y & 0xffffu), skind, snap_scale);
                     f.w = snap_from_bits((unsigned short)(u.y >> 16), skind, snap_scale);
-                    float4 gv = ld4(g);
+                    float4 gv = ld4_nt(g);
                     gv.x = gv.x + (-(((0.5f * (pp.x - pm.x)) * f.x) * prm.ssf));
                     gv.y = gv.y + (-(((0.5f * (pp.y - pm.y)) * f.y) * prm.ssf));
                     gv.z = gv.z + (-(((0.5f * (pp.z - pm.z)) * f.z) * prm.ssf));
                     gv.w = gv.w + (-(((0.5f * (pp.w - pm.w)) * f.w) * prm.ssf));
-                    st4(g, gv);
+                    st4_nt(g, gv);
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         if (xl + i < t.xe) {
                             const float f = snap_load(snap_slot, mi + i, skind, snap_scale);
-                            g[i] = g[i] + (-(((0.5f * (ppv[i] - pmv[i])) * f) * prm.ssf));
+                            st_nt(g + i, ld_nt(g + i) + (-(((0.5f * (ppv[i] - pmv[i])) * f) * prm.ssf)));
                         }
                 }
             }
