@@ -12,8 +12,10 @@ that hold at any size (the oracle itself only finishes small grids quickly).
   gradient runs under a checkpoint schedule and the 16-bit ones (~154 GB) are
   stored.
 * Marine 64-shot gradient, nx=2000 nz=400 nt=5000: the shot grouping of the
-  sweeps does not change a bit, and the exact discrete adjoint passes the
-  dot-product test (normal-operator form) on a full-size shot window.
+  sweeps does not change a bit.
+* The exact discrete adjoint passes the dot-product test (normal-operator
+  form) on the widest shot window of the 2000x400 nt=5000 and of the
+  6000x1000 nt=15000 workloads.
 """
 import gc
 
@@ -145,14 +147,25 @@ def test_marine_64_shot_groups_full_size():
     assert m0 == m1 and np.array_equal(g0, g1)
 
 
-def test_marine_full_window_exact_dot_product():
+@pytest.mark.parametrize("workload, bound", [("marine_2000x400_nt5000", 1e-4), ("checkpoint_6000x1000_nt15000", 5e-4)])
+def test_full_window_exact_dot_product(workload, bound):
     """Dot-product test in its normal-operator form, <F s, F s> = <s, F^T (F s)>,
-    with the shot's Ricker wavelet as s (the FP32 oracle gives 2.8e-6 on this
-    window, FP64 6e-15).  White noise is no test input for an FP32 pair this
-    long: injecting N(0,1) samples every step, the FP32 forward energy drifts
-    1.1 % from FP64 over the 5000 steps and the identity holds to 8.2e-3 only
-    -- on this GPU and, bit for bit, in the FP32 oracle (FP64: 2.5e-13)."""
-    acq = synth.marine_64shot_2000x400()
+    on the widest shot window of a full-size workload, with the shot's Ricker
+    wavelet as s.  The bound is FP32 rounding, not the operator pair:
+    * 2000x400, nt=5000: the FP32 oracle gives 2.8e-6 on this window, FP64 6e-15.
+    * 6000x1000, nt=15000: up to 1e-6 through nt=10000, then 2.9e-5 (W=1613)
+      and 1.4e-4 (W=2411) at nt=15000; on the W=1613 window the FP32 oracle
+      gives the GPU's <Fs,Fs> and <s,F^T Fs> to all printed digits (its
+      seismograms equal the GPU's bit for bit), the FP64 oracle 2.0e-14, and
+      the FP32 seismograms themselves sit 4.2e-4 from FP64 after 15000 steps.
+    White noise is no test input for an FP32 pair this long: injecting N(0,1)
+    samples every step, the FP32 forward energy drifts 1.1 % from FP64 over
+    5000 steps and the identity holds to 8.2e-3 only -- on this GPU and, bit
+    for bit, in the FP32 oracle (FP64: 2.5e-13)."""
+    if workload.startswith("marine"):
+        acq = synth.marine_64shot_2000x400()
+    else:
+        acq = synth.checkpoint_6000x1000(nshots=4)
     i = int(np.argmax([s["w"] for s in acq.shots]))
     shot = acq.shots[i]
     s = acq.wavelets[i]
@@ -170,4 +183,4 @@ def test_marine_full_window_exact_dot_product():
     print(f"\nexact dot-product test, W={shot['w']} nt={acq.nt}: <Fs,Fs>={lhs:.9e} <s,F^T Fs>={rhs:.9e} "
           f"rel {abs(lhs - rhs) / abs(lhs):.1e}")
     assert lhs > 0
-    assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
+    assert abs(lhs - rhs) <= bound * abs(lhs)
