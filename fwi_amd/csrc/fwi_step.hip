@@ -156,8 +156,12 @@ __device__ __forceinline__ void snap_store(GMEM float* store, GMEM void* slot, s
         st_nt(static_cast<GMEM unsigned short*>(slot) + mi, to_bf16_bits(f));
         return;
     }
-    st_nt(store + mi, f);
-    vmax = fmaxf(vmax, fabsf(f));
+    if (kind == SNAP_F16) {                  // staging, read back right after the step by the quantize pass
+        store[mi] = f;
+        vmax = fmaxf(vmax, fabsf(f));
+    } else {
+        st_nt(store + mi, f);
+    }
 }
 
 // Folds the wavefront's largest stored |value| into the slot maximum (float
@@ -814,9 +818,11 @@ __device__ __forceinline__ void interior_task(const StepParams& prm, const Task&
                         const unsigned int hi = to_bf16_bits(f.z) | ((unsigned int)to_bf16_bits(f.w) << 16);
                         const u2v w = {lo, hi};
                         st_nt(reinterpret_cast<GMEM u2v*>(static_cast<GMEM unsigned short*>(snap_slot) + mi), w);
+                    } else if (skind == SNAP_F16) {
+                        st4(store + mi, f);             // staging, read back by the quantize pass
+                        vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
                     } else {
                         st4_nt(store + mi, f);
-                        vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fmaxf(fabsf(f.z), fabsf(f.w))));
                     }
                 } else {
 #pragma unroll
