@@ -6,7 +6,9 @@
   not divide the slot count).
 * Graph-captured sweeps replay the same launches: bit-identical to direct
   launches, on every replay, with other work on the stream in between.
-* BF16 / FP16 snapshots with per-snapshot power-of-two max-abs scaling: the
+* BF16 / FP16 snapshots: bit-exact against the oracle run on its FP32
+  snapshots quantised the same way (BF16 rounded to nearest even, FP16 with
+  the per-snapshot power-of-two max-abs scale).  Against FP32 snapshots the
   FP16 gradient stays within 1e-3 relative L2 (cosine >= 0.9999) of the
   FP32-snapshot gradient.  BF16 keeps 8 significant bits, so each stored value
   carries up to 2^-9 relative rounding noise, which scaling cannot reduce; its
@@ -120,6 +122,71 @@ def test_16bit_snapshots_match_fp32(kind, scale_src):
     c16.set_schedule(13)
     _, g16c, _ = _grad(c16, full)
     assert np.array_equal(g16c, g16)
+
+
+def _bf16_round(a):
+    """float32 -> bfloat16 -> float32, round to nearest even (no NaNs here)."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def _f16_quantise(slot):
+    """One FP16 snapshot slot as stored and read back: power-of-two scale that
+    maps the slot's largest |value| into [2^14, 2^15), half rounding to
+    nearest even, scale applied back."""
+    mx = np.float32(np.abs(slot).max())
+    sc = np.float32(1.0)
+    if mx > 0:
+        _, e = np.frexp(mx)
+        sc = np.float32(np.ldexp(np.float32(1.0), int(e) - 15))
+    inv = np.float32(1.0) / sc
+    return ((slot * inv).astype(np.float16).astype(np.float32) * sc).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", [fwi_amd.SNAP_BF16, fwi_amd.SNAP_F16])
+@pytest.mark.parametrize("scale_src", [1.0, 3.0e4])
+def test_16bit_snapshots_bit_exact_vs_quantised_oracle(kind, scale_src):
+    """16-bit snapshot gradients are not only close to the FP32 ones: each
+    shot's imaging kernel equals, bit for bit, the oracle's adjoint run on the
+    oracle's FP32 snapshots quantised the same way (BF16: rounded to nearest
+    even; FP16: per-shot, per-slot power-of-two scale from the slot maximum).
+    scale_src = 3e4 puts the snapshot values beyond the half-precision range."""
+    rng = np.random.default_rng(17)
+    ny, nx, nt, ss = 36, 300, 200, 10
+    d, dt = np.float32(20.0), np.float32(0.002)
+    vp = (1500 + 1500 * rng.random((ny, nx))).astype(np.float32)
+    shots, wav = [], []
+    for w, x0 in ((120, 10), (200, 90), (60, 230)):
+        nr = 9
+        shots.append(dict(x0=x0, w=w, src_x=[w // 2], src_y=[4], rec_x=rng.integers(1, w + 1, nr).tolist(),
+                          rec_y=rng.integers(1, ny + 1, nr).tolist()))
+        wav.append((synth.ricker(12.0, nt, float(dt)) * np.float32(scale_src))[:, None].astype(np.float32))
+    ctx = fwi_amd.Context(ny, nx, d, dt, nt, ss, dpml=20, free_surface=0, snap_kind=kind)
+    ctx.set_shots(shots)
+    ctx.set_model(vp)
+    ctx.set_sources(wav)
+    ctx.forward(store_snapshots=True)
+    obs = [np.ascontiguousarray(np.roll(a, 2, axis=0)) for a in ctx.synthetic()]
+    ctx.set_observed(obs)
+    ctx.misfit()
+    ctx.adjoint()
+    kers = ctx.kernels()
+    ctx.close()
+    for i, (s, w, o, k) in enumerate(zip(shots, wav, obs, kers)):
+        sub = np.asfortranarray(vp[:, s["x0"]:s["x0"] + s["w"]])
+        args = (s["src_x"], s["src_y"], s["rec_x"], s["rec_y"])
+        O = rb.oracle_solver("f32", 1, *args, w, None, d, dt, sub, ss, nt // ss, nt, free_surface=0)
+        _, adj = rb.oracle_misfit(O.rec, o, float(dt))
+        Fq = O.fields.copy(order="F")
+        for m in range(1, nt // ss + 1):
+            sl = O.fields[:, m * s["w"]:(m + 1) * s["w"]]
+            Fq[:, m * s["w"]:(m + 1) * s["w"]] = _bf16_round(sl) if kind == fwi_amd.SNAP_BF16 else _f16_quantise(sl)
+        Oq = rb.oracle_solver("f32", 2, *args, w, adj, d, dt, sub, ss, nt // ss, nt, fields=Fq, free_surface=0)
+        O32 = rb.oracle_solver("f32", 2, *args, w, adj, d, dt, sub, ss, nt // ss, nt, fields=O.fields,
+                               free_surface=0)
+        assert not np.array_equal(Oq.kernel, O32.kernel)        # the quantisation is visible
+        assert np.array_equal(k, Oq.kernel), f"shot {i}: max diff {np.abs(k - Oq.kernel).max()}"
 
 
 @pytest.mark.skipif(not rb.ref_available("fs0_r8"), reason="reference FP64 build not present")
