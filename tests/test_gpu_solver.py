@@ -172,7 +172,7 @@ def test_batched_windows_vs_oracle(rows):
     g_o *= scale
     for i in range(nx):
         g_o[:water[i], i] = 0.0
-    assert np.allclose(grad, g_o, rtol=1e-12, atol=1e-300)
+    assert np.array_equal(grad, g_o)      # FP64 sum in shot order, then the scale: same operations
 
 
 @pytest.mark.parametrize("dpml,nx,fs", [
