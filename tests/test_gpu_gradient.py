@@ -2,7 +2,8 @@
 and adjoint source, adjoint with the imaging condition, FP64 shot sum.
 
 Tolerances: per-shot seismograms, adjoint sources and imaging kernels are
-bit-exact against the reference (golden vectors) and the oracle.  The GPU
+bit-exact against the reference (golden vectors, including two shots of the
+64-shot benchmark gradient at its full size) and the oracle.  The GPU
 sums the reference's FP32 misfit terms in FP64 and must equal the FP64 sum of
 those terms to 1e-12 relative; the reference's own sequential FP32 sum is
 stated beside it (8e-5 away on the 400x150 case).  FP64 shot sums agree to
@@ -84,6 +85,35 @@ def test_single_shot_gradient_matches_reference_golden():
     print(f"\nsingle shot misfit: GPU {misfit:.12e}, FP64 sum of FP32 terms {m64:.12e}, "
           f"reference FP32 sequential sum {ref32:.12e} (rel. dev. {abs(ref32 - m64) / m64:.1e})")
     assert abs(ref32 - m64) <= 1e-3 * m64
+
+
+def test_marine_64_shot_full_size_matches_reference_golden():
+    """The benchmark gradient itself: all 64 shots at full size (nx=2000 nz=400,
+    nt=5000) in one context with the benchmark's decomposition (4-row interior
+    and 2-row frame work units, 2 shot groups).  For the widest and the median
+    shot, the observed and synthetic seismograms, the adjoint source and the
+    imaging kernel equal bit for bit what the reference's own Fortran solver
+    computes for that shot (tests/golden/make_marine_fullsize.py: SHA-256 of
+    the full arrays, plus decimated copies)."""
+    g = np.load(os.path.join(GOLD, "marine64_nt5000_fs0_r4.npz"), allow_pickle=False)
+    acq = synth.marine_64shot_2000x400()
+    ctx = _ctx(acq, rows=4)
+    assert ctx.shot_groups() == 2 and ctx.frame_rows() == 2
+    obs = ctx.observed()
+    ctx.gradient(1.0 / len(acq.shots), acq.water_rows)
+    syn, adj, kers = ctx.synthetic(), ctx.adjoint_sources(), ctx.kernels()
+    ctx.close()
+    for k in range(int(g["nshots"])):
+        i = int(g[f"s{k}_shot"])
+        assert acq.shots[i]["w"] == int(g[f"s{k}_w"])
+        assert np.array_equal(syn[i][::25, ::16], g[f"s{k}_rec_dec"])
+        assert np.array_equal(kers[i][::4, ::16], g[f"s{k}_kernel_dec"])
+        assert _sha(obs[i]) == str(g[f"s{k}_obs_sha"])
+        assert _sha(syn[i]) == str(g[f"s{k}_rec_sha"])
+        assert _sha(adj[i]) == str(g[f"s{k}_adj_sha"])
+        assert _sha(kers[i]) == str(g[f"s{k}_kernel_sha"])
+        print(f"\nshot {i} (W={acq.shots[i]['w']}, nt={acq.nt}): seismograms, adjoint source and imaging kernel "
+              f"bit-identical to the reference solver")
 
 
 @pytest.mark.skipif(not rb.ref_available("fs0_r8"), reason="reference FP64 build not present")
