@@ -18,6 +18,8 @@ that hold at any size (the oracle itself only finishes small grids quickly).
   6000x1000 nt=15000 workloads.
 """
 import gc
+import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -26,6 +28,7 @@ import fwi_amd
 from fwi_amd import synth
 
 pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _rel(a, b):
@@ -74,8 +77,18 @@ def test_checkpoint_6000x1000_single_shot_matches_stored():
     assert m0 == m1 and np.array_equal(g0, g1)
 
 
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes(order="F")).hexdigest()
+
+
 def test_checkpoint_6000x1000_16_shots_under_64GB():
+    """16 shots under a 64 GB snapshot budget (194 GiB if stored): two
+    checkpoint spacings give the same gradient bit for bit, and the narrowest
+    shot's observed and synthetic seismograms, adjoint source and imaging
+    kernel equal the reference's own Fortran solver run on that shot
+    (tests/golden/make_checkpoint_fullsize.py)."""
     acq = synth.checkpoint_6000x1000(nshots=16)
+    gold = np.load(os.path.join(GOLD, "checkpoint16_nt15000_fs0_r4.npz"), allow_pickle=False)
     budget = 64 << 30
     out = []
     seg = None
@@ -91,6 +104,18 @@ def test_checkpoint_6000x1000_16_shots_under_64GB():
         print(f"\n16 shots, 6000x1000 nt=15000: checkpoints every {seg} snapshots, {need / 2**30:.1f} GiB of "
               f"snapshots and checkpoints (stored: {ctx.schedule_bytes(0) / 2**30:.0f} GiB)")
         assert need <= budget
+        if run == 0:
+            i = int(gold["shot"])
+            assert acq.shots[i]["w"] == int(gold["w"])
+            obs_i = ctx.observed()[i].copy()
+            ctx.gradient(1.0 / 16, acq.water_rows)
+            syn_i, adj_i = ctx.synthetic()[i].copy(), ctx.adjoint_sources()[i].copy()
+            ker_i = ctx.kernels()[i]
+            assert np.array_equal(syn_i[::50, ::16], gold["rec_dec"])
+            assert np.array_equal(ker_i[::8, ::16], gold["kernel_dec"])
+            assert _sha(obs_i) == str(gold["obs_sha"]) and _sha(syn_i) == str(gold["rec_sha"])
+            assert _sha(adj_i) == str(gold["adj_sha"]) and _sha(ker_i) == str(gold["kernel_sha"])
+            print(f"shot {i} (W={acq.shots[i]['w']}): checkpointed outputs bit-identical to the reference solver")
         out.append(_gradient(ctx, acq, 16))
     (m0, g0), (m1, g1) = out
     assert np.abs(g0).max() > 0 and np.isfinite(g0).all()
