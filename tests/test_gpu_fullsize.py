@@ -10,7 +10,8 @@ that hold at any size (the oracle itself only finishes small grids quickly).
   of the FP32-snapshot gradient; the distances are printed.  With all 32
   shots the FP32 snapshots (~308 GB) exceed the GPU's memory, so the FP32
   gradient runs under a checkpoint schedule and the 16-bit ones (~154 GB) are
-  stored.
+  stored; shot 0 of the checkpointed FP32 run equals the reference's own
+  solver bit for bit (tests/golden/make_fullsize.py).
 * Marine 64-shot gradient, nx=2000 nz=400 nt=5000: the shot grouping of the
   sweeps does not change a bit.
 * The exact discrete adjoint passes the dot-product test (normal-operator
@@ -52,6 +53,11 @@ def _context(acq, shots, wav, snap_kind=fwi_amd.SNAP_F32):
     return ctx
 
 
+def _gradient_keep(ctx, acq, nshots):
+    ctx.gradient(1.0 / nshots, acq.water_rows)
+    return ctx.get_gradient()
+
+
 def _gradient(ctx, acq, nshots):
     m = ctx.gradient(1.0 / nshots, acq.water_rows)
     g = ctx.get_gradient()
@@ -86,7 +92,7 @@ def test_checkpoint_6000x1000_16_shots_under_64GB():
     checkpoint spacings give the same gradient bit for bit, and the narrowest
     shot's observed and synthetic seismograms, adjoint source and imaging
     kernel equal the reference's own Fortran solver run on that shot
-    (tests/golden/make_checkpoint_fullsize.py)."""
+    (tests/golden/make_fullsize.py)."""
     acq = synth.checkpoint_6000x1000(nshots=16)
     gold = np.load(os.path.join(GOLD, "checkpoint16_nt15000_fs0_r4.npz"), allow_pickle=False)
     budget = 64 << 30
@@ -105,16 +111,16 @@ def test_checkpoint_6000x1000_16_shots_under_64GB():
               f"snapshots and checkpoints (stored: {ctx.schedule_bytes(0) / 2**30:.0f} GiB)")
         assert need <= budget
         if run == 0:
-            i = int(gold["shot"])
-            assert acq.shots[i]["w"] == int(gold["w"])
+            i = int(gold["s0_shot"])
+            assert acq.shots[i]["w"] == int(gold["s0_w"])
             obs_i = ctx.observed()[i].copy()
             ctx.gradient(1.0 / 16, acq.water_rows)
             syn_i, adj_i = ctx.synthetic()[i].copy(), ctx.adjoint_sources()[i].copy()
             ker_i = ctx.kernels()[i]
-            assert np.array_equal(syn_i[::50, ::16], gold["rec_dec"])
-            assert np.array_equal(ker_i[::8, ::16], gold["kernel_dec"])
-            assert _sha(obs_i) == str(gold["obs_sha"]) and _sha(syn_i) == str(gold["rec_sha"])
-            assert _sha(adj_i) == str(gold["adj_sha"]) and _sha(ker_i) == str(gold["kernel_sha"])
+            assert np.array_equal(syn_i[::50, ::16], gold["s0_rec_dec"])
+            assert np.array_equal(ker_i[::8, ::16], gold["s0_kernel_dec"])
+            assert _sha(obs_i) == str(gold["s0_obs_sha"]) and _sha(syn_i) == str(gold["s0_rec_sha"])
+            assert _sha(adj_i) == str(gold["s0_adj_sha"]) and _sha(ker_i) == str(gold["s0_kernel_sha"])
             print(f"shot {i} (W={acq.shots[i]['w']}): checkpointed outputs bit-identical to the reference solver")
         out.append(_gradient(ctx, acq, 16))
     (m0, g0), (m1, g1) = out
@@ -147,9 +153,19 @@ def test_precision_4000x800_32_shots_checkpointed_fp32_vs_16bit():
     assert stored > budget and seg > 0
     ctx.set_schedule(seg)
     need = ctx.schedule_bytes(seg)
-    g32 = _gradient(ctx, acq, n)[1]
+    # shot 0 of the checkpointed FP32 run against the reference's own solver
+    gold = np.load(os.path.join(GOLD, "precision32_nt10000_fs0_r4.npz"), allow_pickle=False)
+    i = int(gold["s0_shot"])
+    obs_i = ctx.observed()[i].copy()
+    g32 = _gradient_keep(ctx, acq, n)
+    syn_i, adj_i, ker_i = ctx.synthetic()[i].copy(), ctx.adjoint_sources()[i].copy(), ctx.kernels()[i]
+    ctx.close()
+    gc.collect()
+    assert np.array_equal(syn_i[::50, ::16], gold["s0_rec_dec"]) and np.array_equal(ker_i[::8, ::16], gold["s0_kernel_dec"])
+    assert _sha(obs_i) == str(gold["s0_obs_sha"]) and _sha(syn_i) == str(gold["s0_rec_sha"])
+    assert _sha(adj_i) == str(gold["s0_adj_sha"]) and _sha(ker_i) == str(gold["s0_kernel_sha"])
     print(f"\n4000x800 nt=10000, 32 shots: FP32 snapshots would take {stored / 2**30:.0f} GiB; checkpoints every "
-          f"{seg} snapshots in {need / 2**30:.1f} GiB")
+          f"{seg} snapshots in {need / 2**30:.1f} GiB; shot {i} bit-identical to the reference solver")
     assert np.abs(g32).max() > 0 and np.isfinite(g32).all()
     for kind, bound in ((fwi_amd.SNAP_F16, 1e-3), (fwi_amd.SNAP_BF16, 2e-3)):
         ctx = _context(acq, acq.shots, acq.wavelets, snap_kind=kind)

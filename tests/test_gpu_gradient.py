@@ -93,7 +93,7 @@ def test_marine_64_shot_full_size_matches_reference_golden():
     and 2-row frame work units, 2 shot groups).  For the widest and the median
     shot, the observed and synthetic seismograms, the adjoint source and the
     imaging kernel equal bit for bit what the reference's own Fortran solver
-    computes for that shot (tests/golden/make_marine_fullsize.py: SHA-256 of
+    computes for that shot (tests/golden/make_fullsize.py: SHA-256 of
     the full arrays, plus decimated copies)."""
     g = np.load(os.path.join(GOLD, "marine64_nt5000_fs0_r4.npz"), allow_pickle=False)
     acq = synth.marine_64shot_2000x400()
