@@ -1168,7 +1168,7 @@ int fwi_set_shots(fwi_ctx* c, int nshots, const fwi_shot* shots) {
 // c2 of every shot window and of the shared interior copies from d_vp.
 int rebuild_c2(fwi_ctx* c) {
     const fwi_config& cf = c->cfg;
-    HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, cf.nx, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
+    HIPCHK(fwi::launch_build_c2(c->d_vp, cf.ny, c->d_c2shots, (int)c->shots.size(), c->dn, c->NY, cf.dt, cf.d,
                                 c->stream));
     HIPCHK(fwi::launch_build_c2_shared(c->d_vp, cf.ny, cf.nx, cf.dt, cf.d, c->d_c2s, c->Pm, c->c2s_stride, c->stream));
     return 0;

@@ -64,7 +64,7 @@ hipError_t launch_apply_step(const double* search, float* lam0, float* vp, size_
 // lam0 = 1000 * vp^2  (auxiliar.f90:209-212)
 hipError_t launch_base_model(const float* vp, float* lam0, size_t n, hipStream_t st);
 
-hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
+hipError_t launch_build_c2(const float* vp, int ny, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st);
 // Four copies of the model's c2 (copy q shifted right by q columns, row pitch
 // Pm, 4 floats of front padding) for the aligned interior loads of any window.

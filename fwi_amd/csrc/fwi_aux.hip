@@ -12,8 +12,8 @@ namespace {
 // c2 = c^2 dt^2 / d^2 on the padded grid of every shot, with the velocity
 // extended by nearest-edge clamping into the CPML frame
 // (solver_acoustic_SS.f90:271-292).  vp is the full model, column-major (ny, nx).
-__global__ void build_c2_kernel(const float* __restrict__ vp, int ny, int nxm, const C2Shot* __restrict__ shots,
-                                int dn, int NY, float dt, float d) {
+__global__ void build_c2_kernel(const float* __restrict__ vp, int ny, const C2Shot* __restrict__ shots, int dn, int NY,
+                                float dt, float d) {
     const C2Shot sh = shots[blockIdx.y];
     const size_t n = (size_t)NY * sh.P;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -335,10 +335,10 @@ hipError_t launch_copy16(void* dst, const void* src, size_t n16, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_build_c2(const float* vp, int ny, int nxm, const C2Shot* shots, int nshots, int dn, int NY,
+hipError_t launch_build_c2(const float* vp, int ny, const C2Shot* shots, int nshots, int dn, int NY,
                            float dt, float d, hipStream_t st) {
     if (nshots == 0) return hipSuccess;
-    hipLaunchKernelGGL(build_c2_kernel, dim3(256, nshots), dim3(256), 0, st, vp, ny, nxm, shots, dn, NY, dt, d);
+    hipLaunchKernelGGL(build_c2_kernel, dim3(256, nshots), dim3(256), 0, st, vp, ny, shots, dn, NY, dt, d);
     return hipGetLastError();
 }
 
