@@ -251,15 +251,17 @@ long long fwi_schedule_bytes(fwi_ctx* ctx, int segment_slots);
 /* Autotuner (no counterpart in the reference): times `steps` forward steps of
  * the context's own shots for each candidate number of rows per work unit
  * (and each shot-group count 1..4 and the cache-sized count +-1, unless
- * fwi_set_shot_groups fixed one), then the frame work units' height,
- * keeps the fastest and rebuilds the task list.  Call it right after fwi_set_shots: it re-creates the shot state,
- * so the model, sources and data are uploaded afterwards.  Returns the rows
- * chosen and the measured time per step. */
+ * fwi_set_shot_groups fixed one), then the frame work units' height, keeps
+ * the fastest and rebuilds the task list.  Call it right after
+ * fwi_set_shots: it re-creates the shot state, so the model, sources and
+ * data are uploaded afterwards.  Returns the rows chosen and the measured
+ * time per step. */
 int fwi_autotune(fwi_ctx* ctx, int steps, int* best_rows, double* best_step_us);
 /* Rows per work unit of the current task list (interior work units), and
  * rows per frame work unit (default min(rows, 2): the CPML frame's work is
  * latency-bound, so short work units that run many at once are faster;
- * fwi_autotune picks it after the interior height). */
+ * fwi_autotune picks it after the interior height).  No counterpart in the
+ * reference. */
 int fwi_get_rows_per_task(fwi_ctx* ctx);
 int fwi_get_frame_rows(fwi_ctx* ctx);
 
@@ -269,8 +271,8 @@ int fwi_get_frame_rows(fwi_ctx* ctx);
  * before the next group, so the group's propagation state stays in the
  * GPU's last-level cache between steps.  groups = 0 (default) sizes the
  * groups to 192 MiB of state each; fwi_autotune tries 1 to 4 groups and
- * that count +-1 when the setting is 0.  Results are bit-identical for any grouping.
- * fwi_get_shot_groups returns the number of groups in use. */
+ * that count +-1 when the setting is 0.  Results are bit-identical for any
+ * grouping.  fwi_get_shot_groups returns the number of groups in use. */
 int fwi_set_shot_groups(fwi_ctx* ctx, int groups);
 int fwi_get_shot_groups(fwi_ctx* ctx);
 
